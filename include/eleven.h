@@ -1,0 +1,210 @@
+/*
+ * eleven.h — C ABI of the MI355X-native path-tracing core.
+ *
+ * Plain C: opaque handles, POD structs, pointers and sizes.  No C++ or HIP
+ * types cross this boundary.  Every entry point returns an int status
+ * (EL_OK == 0, negative on error) unless stated otherwise; the message of the
+ * last error on a context is returned by el_last_error().
+ *
+ * Threading: one el_ctx per host thread; a context owns one HIP stream on one
+ * device and starts no host threads of its own.  Calls are synchronous on
+ * return unless stated otherwise.
+ *
+ * Reference interfaces replaced (paths relative to src/tfg-pathtracer/):
+ *   renderSetup(Scene*)                       kernel.h:80,  kernel.cu:566-661
+ *   renderCuda(Scene*, int sampleTarget)      kernel.h:78,  kernel.cu:665-686
+ *   getBuffers(RenderData&, int*, int)        kernel.h:82,  kernel.cu:688-710
+ *   getSamples()                              kernel.h:84,  kernel.cu:712-724
+ *   loadScene(std::string)                    SceneLoader.hpp:7-143
+ *   BVH::transverse (closest hit)             BVH.hpp:120-157, kernel.cu:152-173
+ */
+#ifndef ELEVEN_H
+#define ELEVEN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define EL_OK          0
+#define EL_ERR_ARG    -1   /* bad argument (null pointer, zero size, ...)   */
+#define EL_ERR_IO     -2   /* missing / unreadable / corrupt input file     */
+#define EL_ERR_HIP    -3   /* HIP runtime error (no device, launch failure) */
+#define EL_ERR_STATE  -4   /* call out of order (e.g. render before upload) */
+#define EL_ERR_RANGE  -5   /* size beyond a documented limit                */
+#define EL_ERR_NOMEM  -6   /* host or device allocation failed              */
+
+/* ---- render modes / flags --------------------------------------------- */
+#define EL_MODE_PLAIN      0  /* one thread per pixel, full path per thread  */
+#define EL_MODE_WAVEFRONT  1  /* per-bounce ray queues, staged kernels       */
+
+#define EL_FLAG_AOV        1u /* also accumulate NORMAL/TANGENT/BITANGENT    */
+#define EL_FLAG_COUNT_TRAVERSAL 2u /* wavefront: tally node fetches and
+                                      triangle tests of the closest-hit stage */
+
+/* Passes, same numbering as the reference's `enum Passes` (kernel.h:7). */
+#define EL_PASS_BEAUTY     0
+#define EL_PASS_DENOISE    1  /* not produced (denoiser is out of scope)     */
+#define EL_PASS_NORMAL     2
+#define EL_PASS_TANGENT    3
+#define EL_PASS_BITANGENT  4
+
+/* ---- scene description (POD mirror of the reference structs) ---------- */
+typedef struct { float x, y, z; } el_float3;           /* Vector.hpp:7-12  */
+
+/* Material.hpp:8-37 field for field (name dropped). */
+typedef struct {
+    int32_t albedo_tex, emission_tex, roughness_tex, metallic_tex,
+            normal_tex, opacity_tex;                   /* -1 = none */
+    el_float3 albedo, emission, opacity;
+    float roughness, metallic, clearcoat_gloss, clearcoat, anisotropic, eta,
+          transmission, specular, specular_tint, sheen_tint, subsurface, sheen;
+} el_material;
+
+/* Texture.hpp:19-33: FP32 RGB interleaved, data[3*(y*width+x)+c]. */
+typedef struct {
+    int32_t width, height;
+    const float* data;
+    float x_tile, y_tile, x_offset, y_offset;
+    int32_t filter;                                    /* 0 nearest, 1 bilinear */
+} el_texture;
+
+typedef struct { el_float3 position, radiance; } el_point_light; /* PointLight.hpp */
+
+/* Camera.hpp:10-23 */
+typedef struct {
+    uint32_t x_res, y_res;
+    float focal_length, sensor_width, sensor_height, aperture, focus_distance;
+    el_float3 rotation;                                /* Euler XYZ, degrees */
+    el_float3 position;
+    int32_t bokeh;
+} el_camera;
+
+/* A whole scene as flat arrays.  The library copies everything it needs
+ * during el_upload_scene; the caller keeps ownership of every pointer. */
+typedef struct {
+    uint32_t tri_count;
+    const float*   vertices;      /* [tri_count][3][3]  Tri::vertices        */
+    const float*   uvs;           /* [tri_count][3][2]  Tri::uv (x, y)       */
+    const float*   normals;       /* [tri_count][3][3]  Tri::normals         */
+    const float*   tangents;      /* [tri_count][3][3]  Tri::tangents        */
+    const float*   tangent_sign;  /* [tri_count]        Tri::tangentsSign    */
+    const int32_t* object_id;     /* [tri_count]        Tri::objectID        */
+    uint32_t mesh_count;
+    const int32_t* mesh_material; /* [mesh_count]       MeshObject::materialID */
+    uint32_t material_count;
+    const el_material* materials;
+    uint32_t texture_count;
+    const el_texture* textures;
+    el_texture hdri;              /* environment map (HDRI::texture)          */
+    uint32_t light_count;
+    const el_point_light* lights;
+    el_camera camera;
+} el_scene_view;
+
+typedef struct { el_float3 origin, direction; } el_ray; /* Ray.hpp:8-17 */
+
+typedef struct {
+    uint32_t spp;          /* samples per pixel added by this call          */
+    uint32_t max_depth;    /* MAXBOUNCES (Definitions.h:9); 0 -> 5          */
+    uint64_t seed;         /* RNG key                                       */
+    int32_t  mode;         /* EL_MODE_*                                     */
+    uint32_t block;        /* threads per block, 0 -> tuned default         */
+    uint32_t queue_cap;    /* wavefront paths in flight, 0 -> default       */
+    uint32_t bvh_width;    /* 4 or 8, 0 -> current                          */
+    uint32_t flags;        /* EL_FLAG_*                                     */
+} el_render_params;
+
+typedef struct {
+    uint64_t rays;             /* BVH queries: camera+extension+shadow       */
+    uint64_t vertices;         /* shaded path vertices (reference kPaths)    */
+    uint64_t paths;            /* paths traced                               */
+    double   render_ms;        /* wall time of the last el_render (device)   */
+    double   extend_ms;        /* summed closest-hit kernel time (wavefront) */
+    double   shadow_ms;        /* summed shadow kernel time (wavefront)      */
+    double   shade_ms;         /* summed shading kernel time (wavefront)     */
+    uint64_t extend_launches;  /* number of closest-hit launches timed       */
+    uint64_t extend_rays;      /* rays processed by those launches           */
+    uint64_t node_fetches;     /* closest-hit node fetches (COUNT_TRAVERSAL) */
+    uint64_t tri_tests;        /* closest-hit triangle tests (COUNT_TRAVERSAL) */
+    uint64_t node_bytes;       /* size of the device BVH node array          */
+    uint64_t tri_bytes;        /* size of the device triangle stream         */
+    uint32_t bvh_width;
+    uint32_t stack_bound;      /* worst-case traversal stack depth           */
+} el_stats;
+
+/* ---- host-side scene loading (SceneLoader.hpp:7 loadScene) ------------ */
+typedef struct el_scene el_scene;
+
+/* Load <dir>/scene.json, scene.mtl, scene.obj, HDRI/<name>.hdr and the .bmp
+ * maps named in scene.mtl.  Accepts '/' and '\\' separators. */
+int  el_scene_load(const char* dir, el_scene** out);
+/* Flat view of a loaded scene; pointers stay valid until el_scene_free. */
+int  el_scene_view_get(const el_scene* scene, el_scene_view* out);
+/* HDRI importance-sampling data the reference builds in HDRI::generateCDF
+ * (HDRI.hpp:107-128): cdf[w*h+1] and the FP32 radiance sum. */
+int  el_scene_hdri_cdf(const el_scene* scene, const float** cdf, float* radiance_sum);
+void el_scene_free(el_scene* scene);
+const char* el_scene_last_error(void);
+
+/* ---- device context ---------------------------------------------------- */
+typedef struct el_ctx el_ctx;
+
+int  el_create(el_ctx** out, int device);
+void el_destroy(el_ctx* ctx);
+const char* el_last_error(const el_ctx* ctx);
+int  el_device_count(int* out);
+
+/* renderSetup: builds the BVH, uploads the scene, clears the accumulation. */
+int  el_upload_scene(el_ctx* ctx, const el_scene_view* view);
+/* renderCuda: adds params->spp samples per pixel to the accumulation. */
+int  el_render(el_ctx* ctx, const el_render_params* params);
+/* Clears the accumulation buffers and the sample counter. */
+int  el_clear(el_ctx* ctx);
+/* getBuffers/getSamples: mean beauty RGBA [H][W][4] (row 0 = top, as the
+ * reference's idx = W*(H-y-1)+x), per-pixel accepted sample counts [H][W],
+ * and the ray / vertex tallies.  Any output pointer may be NULL. */
+int  el_read_passes(el_ctx* ctx, float* rgba, uint32_t* samples,
+                    uint64_t* rays, uint64_t* vertices);
+/* AOV pass (EL_PASS_NORMAL/TANGENT/BITANGENT) as mean RGBA [H][W][4]. */
+int  el_read_aov(el_ctx* ctx, int pass, float* rgba);
+int  el_get_stats(el_ctx* ctx, el_stats* out);
+
+/* Replace material parameters on the device without re-uploading geometry
+ * (used by the inverse-rendering loop). */
+int  el_set_materials(el_ctx* ctx, const el_material* materials, uint32_t count);
+/* Replace the texels of texture `tex_id` (same size as uploaded). */
+int  el_set_texture(el_ctx* ctx, int32_t tex_id, const float* rgb);
+
+/* Closest hit for n rays: tri = original triangle index or -1, t = hit
+ * distance along the (caller-normalised) direction, uv = barycentrics
+ * (may be NULL).  Closest means minimum t, ties to the lower triangle
+ * index.  With `device_ptrs` != 0 all pointers are device pointers and the
+ * call is ordered on the context's stream (still synchronous on return). */
+int  el_trace_closest(el_ctx* ctx, const el_ray* rays, uint32_t n,
+                      int32_t* tri, float* t, float* uv, int device_ptrs);
+/* Any hit with 0 <= t < tmax[i]: occluded[i] = 1 / 0. */
+int  el_trace_any(el_ctx* ctx, const el_ray* rays, const float* tmax,
+                  uint32_t n, uint8_t* occluded, int device_ptrs);
+/* Node fetches and triangle tests of the closest-hit traversal of n rays
+ * (same kernel, counting variant): the algorithmic traffic of a batch. */
+int  el_trace_count(el_ctx* ctx, const el_ray* rays, uint32_t n,
+                    uint64_t* node_fetches, uint64_t* tri_tests, int device_ptrs);
+/* Mean duration (ms) of the closest-hit kernel over the last `iters`
+ * launches of el_trace_bench (HIP events on the context's stream). */
+int  el_trace_bench(el_ctx* ctx, const el_ray* d_rays, uint32_t n,
+                    int32_t* d_tri, float* d_t, int iters, double* mean_ms);
+
+/* Device memory helpers for callers without their own allocator. */
+int  el_device_alloc(el_ctx* ctx, size_t bytes, void** out);
+int  el_device_free(el_ctx* ctx, void* ptr);
+int  el_memcpy_htod(el_ctx* ctx, void* dst, const void* src, size_t bytes);
+int  el_memcpy_dtoh(el_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ELEVEN_H */

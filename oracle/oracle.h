@@ -1,0 +1,100 @@
+/*
+ * oracle.h — CPU restatement of the reference path tracer, used ONLY as the
+ * checker by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ * It is test infrastructure: the product (libeleven.so, the CLI) never links,
+ * loads or calls it.
+ *
+ * Every function restates the reference (src/tfg-pathtracer/, read as text)
+ * and cites the lines it follows.  Nothing here is copied from the reference;
+ * the reference cannot be built in this image (its headers need the CUDA
+ * runtime header, which is absent, and stand-ins are not allowed), so the
+ * oracle is pinned by closed forms, brute force, invariants and exhaustive
+ * checks in tests/test_oracle_*.py.
+ *
+ * Scene input uses the product's plain-C el_scene_view struct (a POD layout
+ * shared with include/eleven.h; no product code is called).
+ */
+#ifndef ELEVEN_ORACLE_H
+#define ELEVEN_ORACLE_H
+
+#include <stdint.h>
+
+#include "../include/eleven.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Tri::hit (Tri.hpp:38-68): returns 1 on hit; t, u, v out. */
+int or_tri_hit(const float* v9, const float* o3, const float* d3, float* t, float* u, float* v);
+
+/* Brute-force closest hit over all triangles: min t, ties to the lower index
+ * (the build's definition of "closest").  tri = -1 on a miss. */
+void or_closest_brute(const float* verts, int n, const float* rays, int nrays, int32_t* tri, float* t, float* u,
+                      float* v, int threads);
+
+/* Reference-faithful BVH: complete binary tree of depth `depth` with
+ * `bins`-bin SAH splits (BVH.hpp:187-211, 291-330, 373-460, 492-578), 44-byte
+ * nodes in DFS order and a triIndices array.  Returns an opaque handle. */
+void* or_refbvh_build(const float* verts, int n, int depth, int bins);
+void  or_refbvh_free(void* h);
+/* Traversal as BVH.hpp:120-157 (left-first, right pushed, right leaf tested
+ * first at depth-1, no t culling) with the closest rule above. */
+void  or_refbvh_closest(void* h, const float* rays, int nrays, int32_t* tri, float* t, int threads);
+/* Any hit with 0 <= t < tmax[i]. */
+void  or_refbvh_any(void* h, const float* rays, const float* tmax, int nrays, uint8_t* occ, int threads);
+int   or_refbvh_node_count(void* h);
+
+/* Philox4x32-10 (Salmon et al. SC'11): out[4] for counter c[4], key k[2]. */
+void or_philox(const uint32_t* c, const uint32_t* k, uint32_t* out);
+
+/* Disney BRDF pieces (Disney.hpp:41-253).  hd: 12 scalars in HitData order
+ * (metallic, roughness, clearcoatGloss, clearcoat, anisotropic, eta,
+ * transmission, specular, specularTint, sheenTint, subsurface, sheen) then
+ * emission[3], albedo[3], normal[3]; `dir` is the incoming ray direction.
+ * precision: 0 = FP32 as the reference, 1 = FP64. */
+void   or_disney_eval(const double* hd, const double* dir, const double* L, int precision, double* out3);
+double or_disney_pdf(const double* hd, const double* dir, const double* L, int precision);
+void   or_disney_sample(const double* hd, const double* dir, double r1, double r2, double r3, int precision,
+                        double* out3);
+void   or_cosine_sample_hemisphere(double u1, double u2, double* out3);
+void   or_importance_sample_ggx(double rgh, double r1, double r2, double* out3);
+void   or_uniform_circle(double u1, double u2, double u3, double* x, double* y);
+
+/* HDRI (HDRI.hpp:107-162): CDF and radiance sum at construction, the
+ * reference binarySearch, and the pdf of pixel (x, y). */
+void  or_hdri_cdf(const float* rgb, int w, int h, float* cdf, float* radiance_sum);
+int   or_hdri_search(const float* cdf, float r, int length);
+float or_hdri_pdf(const float* rgb, int w, int h, float x_offset, float radiance_sum, int x, int y);
+
+/* calculateCameraRay (kernel.cu:260-337). */
+void or_camera_ray(const el_camera* cam, int x, int y, const float* r5, float* o3, float* d3);
+
+/* Full estimator (kernel.cu:369-481).
+ *   precision   0: FP32 as the reference, 1: FP64 shading and accumulation
+ *   hdri_mode   0: reference binarySearch over the CDF, 1: alias table
+ *               (alias_prob/alias_idx, bin from the 32-bit shade-r1 draw and
+ *               acceptance from the extra draw, as the GPU)
+ *   shadow_mode 0: reference point-light visibility (closest hit, smooth
+ *               shading position closer than the light); 1: any hit with
+ *               t < dist - 0.001 (the build's definition)
+ * Pixels [x0,x1) x [y0,y1) (image coordinates as the kernel's x, y), samples
+ * [s0, s0+spp).  Outputs per output pixel idx = W*(H-y-1)+x of the full image:
+ * sum[3] and sumsq[3] (double) of accepted clamped samples, count, and the
+ * tallies rays / vertices over all paths. */
+typedef struct {
+    uint32_t spp, s0, depth;
+    uint64_t seed;
+    int precision, hdri_mode, shadow_mode, threads;
+    int x0, x1, y0, y1;
+    const float* alias_prob;
+    const int32_t* alias_idx;
+} or_render_params;
+
+int or_render(const el_scene_view* scene, void* refbvh, const or_render_params* p, double* sum, double* sumsq,
+              uint32_t* count, uint64_t* rays, uint64_t* vertices);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

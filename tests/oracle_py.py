@@ -1,0 +1,234 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) — test infrastructure.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+always as the checker, never as the thing measured or shipped.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+ORACLE_SO = ROOT / "oracle" / "liboracle.so"
+
+_lib = None
+
+
+class or_render_params(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("s0", C.c_uint32), ("depth", C.c_uint32), ("seed", C.c_uint64),
+                ("precision", C.c_int), ("hdri_mode", C.c_int), ("shadow_mode", C.c_int), ("threads", C.c_int),
+                ("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int),
+                ("alias_prob", C.POINTER(C.c_float)), ("alias_idx", C.POINTER(C.c_int32))]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            subprocess.run(["make", "-C", str(ROOT / "oracle")], check=True, capture_output=True)
+        L = C.CDLL(str(ORACLE_SO))
+        P = C.c_void_p
+        L.or_tri_hit.argtypes = [P, P, P, P, P, P]
+        L.or_tri_hit.restype = C.c_int
+        L.or_closest_brute.argtypes = [P, C.c_int, P, C.c_int, P, P, P, P, C.c_int]
+        L.or_refbvh_build.argtypes = [P, C.c_int, C.c_int, C.c_int]
+        L.or_refbvh_build.restype = P
+        L.or_refbvh_free.argtypes = [P]
+        L.or_refbvh_closest.argtypes = [P, P, C.c_int, P, P, C.c_int]
+        L.or_refbvh_any.argtypes = [P, P, P, C.c_int, P, C.c_int]
+        L.or_refbvh_node_count.argtypes = [P]
+        L.or_refbvh_node_count.restype = C.c_int
+        L.or_philox.argtypes = [P, P, P]
+        L.or_disney_eval.argtypes = [P, P, P, C.c_int, P]
+        L.or_disney_pdf.argtypes = [P, P, P, C.c_int]
+        L.or_disney_pdf.restype = C.c_double
+        L.or_disney_sample.argtypes = [P, P, C.c_double, C.c_double, C.c_double, C.c_int, P]
+        L.or_cosine_sample_hemisphere.argtypes = [C.c_double, C.c_double, P]
+        L.or_importance_sample_ggx.argtypes = [C.c_double, C.c_double, C.c_double, P]
+        L.or_uniform_circle.argtypes = [C.c_double, C.c_double, C.c_double, P, P]
+        L.or_hdri_cdf.argtypes = [P, C.c_int, C.c_int, P, P]
+        L.or_hdri_search.argtypes = [P, C.c_float, C.c_int]
+        L.or_hdri_search.restype = C.c_int
+        L.or_hdri_pdf.argtypes = [P, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int]
+        L.or_hdri_pdf.restype = C.c_float
+        L.or_camera_ray.argtypes = [P, C.c_int, C.c_int, P, P, P]
+        L.or_render.argtypes = [P, P, C.POINTER(or_render_params), P, P, P, C.POINTER(C.c_uint64),
+                                C.POINTER(C.c_uint64)]
+        L.or_render.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def tri_hit(v9, o, d):
+    t, u, v = C.c_float(), C.c_float(), C.c_float()
+    h = lib().or_tri_hit(_f32(v9).ctypes.data, _f32(o).ctypes.data, _f32(d).ctypes.data, C.byref(t), C.byref(u),
+                         C.byref(v))
+    return bool(h), t.value, u.value, v.value
+
+
+def closest_brute(verts, rays, threads=0):
+    v = _f32(verts).reshape(-1, 9)
+    r = _f32(rays).reshape(-1, 6)
+    n = r.shape[0]
+    tri = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float32)
+    u = np.zeros(n, np.float32)
+    w = np.zeros(n, np.float32)
+    lib().or_closest_brute(v.ctypes.data, v.shape[0], r.ctypes.data, n, tri.ctypes.data, t.ctypes.data,
+                           u.ctypes.data, w.ctypes.data, threads)
+    return tri, t, u, w
+
+
+class RefBVH:
+    """Reference-faithful depth-18, 14-bin SAH binary BVH (BVH.hpp)."""
+
+    def __init__(self, verts, depth=18, bins=14):
+        self.v = _f32(verts).reshape(-1, 9)
+        self.h = lib().or_refbvh_build(self.v.ctypes.data, self.v.shape[0], depth, bins)
+        if not self.h:
+            raise ValueError("bad BVH parameters")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_refbvh_free(self.h)
+            self.h = None
+
+    def closest(self, rays, threads=0):
+        r = _f32(rays).reshape(-1, 6)
+        tri = np.zeros(r.shape[0], np.int32)
+        t = np.zeros(r.shape[0], np.float32)
+        lib().or_refbvh_closest(self.h, r.ctypes.data, r.shape[0], tri.ctypes.data, t.ctypes.data, threads)
+        return tri, t
+
+    def any(self, rays, tmax, threads=0):
+        r = _f32(rays).reshape(-1, 6)
+        tm = _f32(tmax)
+        occ = np.zeros(r.shape[0], np.uint8)
+        lib().or_refbvh_any(self.h, r.ctypes.data, tm.ctypes.data, r.shape[0], occ.ctypes.data, threads)
+        return occ.astype(bool)
+
+
+def philox(counter, key):
+    c = np.ascontiguousarray(counter, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().or_philox(c.ctypes.data, k.ctypes.data, out.ctypes.data)
+    return out
+
+
+HD_FIELDS = ["metallic", "roughness", "clearcoatGloss", "clearcoat", "anisotropic", "eta", "transmission",
+             "specular", "specularTint", "sheenTint", "subsurface", "sheen"]
+
+
+def hitdata(normal=(0, 0, 1), albedo=(0.5, 0.5, 0.5), emission=(0, 0, 0), **kw):
+    """HitData as 21 doubles: 12 scalars (HD_FIELDS order), emission, albedo, normal."""
+    d = {"metallic": 0.0, "roughness": 0.5, "clearcoatGloss": 0.0, "clearcoat": 0.0, "anisotropic": 0.0,
+         "eta": 0.0, "transmission": 0.0, "specular": 0.5, "specularTint": 0.0, "sheenTint": 0.0,
+         "subsurface": 0.0, "sheen": 0.0}
+    d.update(kw)
+    return np.array([d[k] for k in HD_FIELDS] + list(emission) + list(albedo) + list(normal), np.float64)
+
+
+def disney_eval(hd, direction, L, precision=1):
+    out = np.zeros(3)
+    lib().or_disney_eval(np.ascontiguousarray(hd, np.float64).ctypes.data,
+                         np.ascontiguousarray(direction, np.float64).ctypes.data,
+                         np.ascontiguousarray(L, np.float64).ctypes.data, precision, out.ctypes.data)
+    return out
+
+
+def disney_pdf(hd, direction, L, precision=1):
+    return lib().or_disney_pdf(np.ascontiguousarray(hd, np.float64).ctypes.data,
+                               np.ascontiguousarray(direction, np.float64).ctypes.data,
+                               np.ascontiguousarray(L, np.float64).ctypes.data, precision)
+
+
+def disney_sample(hd, direction, r1, r2, r3, precision=1):
+    out = np.zeros(3)
+    lib().or_disney_sample(np.ascontiguousarray(hd, np.float64).ctypes.data,
+                           np.ascontiguousarray(direction, np.float64).ctypes.data, r1, r2, r3, precision,
+                           out.ctypes.data)
+    return out
+
+
+def cosine_sample_hemisphere(u1, u2):
+    out = np.zeros(3)
+    lib().or_cosine_sample_hemisphere(u1, u2, out.ctypes.data)
+    return out
+
+
+def importance_sample_ggx(rgh, r1, r2):
+    out = np.zeros(3)
+    lib().or_importance_sample_ggx(rgh, r1, r2, out.ctypes.data)
+    return out
+
+
+def uniform_circle(u1, u2, u3):
+    x, y = C.c_double(), C.c_double()
+    lib().or_uniform_circle(u1, u2, u3, C.byref(x), C.byref(y))
+    return x.value, y.value
+
+
+def hdri_cdf(rgb):
+    a = _f32(rgb)
+    h, w = a.shape[:2]
+    cdf = np.zeros(w * h + 1, np.float32)
+    rs = C.c_float()
+    lib().or_hdri_cdf(a.ctypes.data, w, h, cdf.ctypes.data, C.byref(rs))
+    return cdf, rs.value
+
+
+def hdri_search(cdf, r, length):
+    return lib().or_hdri_search(_f32(cdf).ctypes.data, float(r), length)
+
+
+def hdri_pdf(rgb, x_offset, radiance_sum, x, y):
+    a = _f32(rgb)
+    h, w = a.shape[:2]
+    return lib().or_hdri_pdf(a.ctypes.data, w, h, x_offset, radiance_sum, x, y)
+
+
+def camera_ray(cam, x, y, r5):
+    o = np.zeros(3, np.float32)
+    d = np.zeros(3, np.float32)
+    lib().or_camera_ray(C.byref(cam), x, y, _f32(r5).ctypes.data, o.ctypes.data, d.ctypes.data)
+    return o, d
+
+
+def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, hdri_mode=0, shadow_mode=1, threads=0,
+           window=None, alias=None):
+    """Oracle render of pixels `window` = (x0, x1, y0, y1) (kernel x, y).
+
+    Returns (sum[H,W,3], sumsq[H,W,3], count[H,W], rays, vertices) in output
+    layout (row 0 = top)."""
+    cam = scene.view.camera
+    W, H = cam.x_res, cam.y_res
+    x0, x1, y0, y1 = window or (0, W, 0, H)
+    p = or_render_params()
+    p.spp, p.s0, p.depth, p.seed = spp, s0, depth, seed
+    p.precision, p.hdri_mode, p.shadow_mode, p.threads = precision, hdri_mode, shadow_mode, threads
+    p.x0, p.x1, p.y0, p.y1 = x0, x1, y0, y1
+    keep = []
+    if hdri_mode == 1:
+        prob, idx = alias
+        prob = _f32(prob)
+        idx = np.ascontiguousarray(idx, np.int32)
+        keep += [prob, idx]
+        p.alias_prob = prob.ctypes.data_as(C.POINTER(C.c_float))
+        p.alias_idx = idx.ctypes.data_as(C.POINTER(C.c_int32))
+    s = np.zeros((H, W, 3))
+    ss = np.zeros((H, W, 3))
+    cnt = np.zeros((H, W), np.uint32)
+    rays, verts = C.c_uint64(), C.c_uint64()
+    rc = lib().or_render(C.byref(scene.view), refbvh.h, C.byref(p), s.ctypes.data, ss.ctypes.data,
+                         cnt.ctypes.data, C.byref(rays), C.byref(verts))
+    if rc != 0:
+        raise ValueError("or_render rejected its arguments")
+    return s, ss, cnt, rays.value, verts.value

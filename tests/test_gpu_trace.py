@@ -1,0 +1,133 @@
+"""Closest-hit parity: the HIP traversal (el_trace_closest) against the CPU
+oracle on identical seeded ray batches.
+
+Bar: the triangle index matches bit-exactly (minimum t, ties to the lower
+index) and the hit distance is bit-identical (the same Moller-Trumbore
+arithmetic, Tri.hpp:38-68, in FP32 without contraction).  The stated
+tolerance for t is 1e-5 relative; identical bits are checked first and the
+tolerance is asserted as the contract.
+"""
+import numpy as np
+import pytest
+
+import eleven
+from eleven import scenegen
+import oracle_py as orc
+import raygen
+
+T_RTOL = 1e-5
+
+
+def _check(tri_g, t_g, tri_o, t_o, label):
+    assert tri_g.shape == tri_o.shape
+    bad = np.nonzero(tri_g != tri_o)[0]
+    assert bad.size == 0, f"{label}: {bad.size} index mismatches, first {bad[:8]} gpu {tri_g[bad[:8]]} oracle {tri_o[bad[:8]]}"
+    hit = tri_o >= 0
+    assert np.all(np.isinf(t_g[~hit]))
+    rel = np.abs(t_g[hit] - t_o[hit]) / np.maximum(np.abs(t_o[hit]), 1e-30)
+    assert rel.max(initial=0) <= T_RTOL, f"{label}: t rel err {rel.max()}"
+    return int(hit.sum()), int(np.sum(t_g[hit] != t_o[hit]))
+
+
+@pytest.fixture(scope="module")
+def c0(scene_root):
+    d = scenegen.sphere_field(scene_root / "c0")
+    return eleven.SceneData.load(d)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("width", [4, 8])
+def test_closest_random_rays_vs_oracle(gpu_ctx, c0, width):
+    print(f"[trace] sphere field, 2^20 random rays, BVH{width}", flush=True)
+    gpu_ctx.upload(c0)
+    gpu_ctx.render(0, bvh_width=width)          # switch width without rendering
+    verts = c0.vertices()
+    rays = raygen.random_rays(verts, 1 << 20, seed=1)
+    tri_g, t_g, _ = gpu_ctx.trace_closest(rays)
+    ref = orc.RefBVH(verts)
+    tri_r, t_r = ref.closest(rays)
+    # where the reference-faithful tree disagrees, brute force decides
+    dis = np.nonzero(tri_r != tri_g)[0]
+    if dis.size:
+        tb, tt, _, _ = orc.closest_brute(verts, rays[dis])
+        tri_r[dis], t_r[dis] = tb, tt
+    nh, ndiff = _check(tri_g, t_g, tri_r, t_r, "random rays")
+    assert nh > 1000
+    assert ndiff == 0, f"{ndiff} hit distances not bit-identical"
+    # exhaustive check on a slice against brute force over all triangles
+    sub = rays[:8192]
+    tb, tt, _, _ = orc.closest_brute(verts, sub)
+    _check(tri_g[:8192], t_g[:8192], tb, tt, "brute-force slice")
+
+
+@pytest.mark.gpu
+def test_closest_camera_and_tie_rays(gpu_ctx, c0):
+    print("[trace] sphere field camera rays (256x256x16) + 4096 tie rays", flush=True)
+    gpu_ctx.upload(c0)
+    verts = c0.vertices()
+    cam = raygen.camera_rays(c0.camera, spp=16, seed=2)
+    ties = raygen.tie_rays(verts, 4096, seed=3)
+    rays = np.concatenate([cam, ties])
+    tri_g, t_g, uv_g = gpu_ctx.trace_closest(rays)
+    ref = orc.RefBVH(verts)
+    tri_r, t_r = ref.closest(rays)
+    dis = np.nonzero(tri_r != tri_g)[0]
+    if dis.size:
+        tb, tt, _, _ = orc.closest_brute(verts, rays[dis])
+        tri_r[dis], t_r[dis] = tb, tt
+    _check(tri_g, t_g, tri_r, t_r, "camera+tie rays")
+    tb, tt, ub, vb = orc.closest_brute(verts, ties)
+    _check(tri_g[-4096:], t_g[-4096:], tb, tt, "tie rays brute force")
+    hit = tb >= 0
+    assert np.array_equal(uv_g[-4096:][hit, 0], ub[hit]) and np.array_equal(uv_g[-4096:][hit, 1], vb[hit])
+
+
+@pytest.mark.gpu
+def test_any_hit_matches_closest(gpu_ctx, c0):
+    print("[trace] any-hit vs closest", flush=True)
+    gpu_ctx.upload(c0)
+    verts = c0.vertices()
+    rays = raygen.random_rays(verts, 1 << 16, seed=4)
+    tri, t, _ = gpu_ctx.trace_closest(rays)
+    rng = np.random.default_rng(5)
+    tmax = rng.uniform(0, 40, size=rays.shape[0]).astype(np.float32)
+    occ = gpu_ctx.trace_any(rays, tmax)
+    expect = (tri >= 0) & (t < tmax)
+    assert np.array_equal(occ, expect)
+
+
+@pytest.mark.gpu
+def test_random_triangle_soup_vs_brute_force(gpu_ctx):
+    print("[trace] random triangle soups vs brute force", flush=True)
+    rng = np.random.default_rng(11)
+    for n, size in [(1, 1.0), (7, 0.5), (300, 0.3), (5000, 0.05)]:
+        c = rng.uniform(-1, 1, size=(n, 1, 3))
+        v = (c + rng.normal(scale=size, size=(n, 3, 3))).astype(np.float32).reshape(n, 9)
+        sc = eleven.SceneData.from_arrays(v, camera=dict(x_res=4, y_res=4))
+        gpu_ctx.upload(sc)
+        rays = raygen.random_rays(v, 20000, seed=n)
+        tri_g, t_g, _ = gpu_ctx.trace_closest(rays)
+        tb, tt, _, _ = orc.closest_brute(v, rays)
+        _check(tri_g, t_g, tb, tt, f"soup n={n}")
+
+
+@pytest.mark.gpu
+def test_duplicate_triangles_tie_to_lower_index(gpu_ctx):
+    print("[trace] duplicated triangles", flush=True)
+    rng = np.random.default_rng(12)
+    base = rng.uniform(-1, 1, size=(50, 9)).astype(np.float32)
+    v = np.concatenate([base, base, base])       # every triangle three times
+    perm = rng.permutation(v.shape[0])
+    v = v[perm]
+    sc = eleven.SceneData.from_arrays(v, camera=dict(x_res=4, y_res=4))
+    gpu_ctx.upload(sc)
+    rays = raygen.random_rays(v, 20000, seed=13)
+    tri_g, t_g, _ = gpu_ctx.trace_closest(rays)
+    tb, tt, _, _ = orc.closest_brute(v, rays)
+    _check(tri_g, t_g, tb, tt, "duplicates")
+    hit = tb >= 0
+    # the winner is the lowest index among the copies
+    bid = perm % 50                              # base triangle of each entry
+    for i in np.nonzero(hit)[0][:200]:
+        copies = np.nonzero(bid == bid[tri_g[i]])[0]
+        assert tri_g[i] == copies.min()

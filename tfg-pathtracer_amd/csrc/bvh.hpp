@@ -1,0 +1,65 @@
+// Wide BVH for the MI355X traversal kernels.
+//
+// Replaces the reference's fixed-depth binary tree (BVH.hpp:29-62, built by
+// buildAux/divideSAH at BVH.hpp:291-460): a binned-SAH binary tree with
+// SAH-terminated leaves is collapsed into W-wide nodes (W = 4 or 8) whose
+// child boxes are stored structure-of-arrays so one node is W*32 bytes of
+// aligned 16-byte loads.  Leaf triangles are stored contiguously in leaf
+// order as three float4 (v0, e1 = v1-v0, e2 = v2-v0, original index).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace eleven {
+
+// Device node layout.  child[i] >= 0: internal node index; child[i] == -1:
+// empty slot (its box is inverted so it never intersects); child[i] <= -2:
+// leaf, k = -child-2, first triangle = k >> 4, triangle count = (k & 15) + 1.
+template <int W>
+struct alignas(16) WideNode {
+    float lox[W], loy[W], loz[W];
+    float hix[W], hiy[W], hiz[W];
+    int32_t child[W];
+    int32_t pad[W];
+};
+static_assert(sizeof(WideNode<4>) == 128, "BVH4 node must be 128 B");
+static_assert(sizeof(WideNode<8>) == 256, "BVH8 node must be 256 B");
+
+constexpr int kMaxLeafTris = 8;        // <= 16 fits the leaf encoding
+constexpr int kLeafFirstLimit = 1 << 27;
+
+inline int32_t encodeLeaf(int first, int count) { return -(2 + ((first << 4) | (count - 1))); }
+
+struct Bvh2Node {
+    float lo[3], hi[3];
+    int left = -1, right = -1;   // internal: children
+    int first = 0, count = 0;    // leaf: range in the leaf-order array
+    bool leaf() const { return count > 0; }
+};
+
+struct Bvh2 {
+    std::vector<Bvh2Node> nodes;   // node 0 is the root
+    std::vector<int> order;        // leaf order -> original triangle index
+};
+
+struct WideBvh {
+    int width = 4;
+    std::vector<unsigned char> nodes;   // nodeCount * sizeof(WideNode<W>)
+    size_t nodeCount = 0;
+    std::vector<float> triStream;       // 12 floats per leaf-order triangle
+    std::vector<int> order;             // leaf order -> original index
+    int stackBound = 0;                 // worst-case traversal stack depth
+    int depth = 0;                      // wide-tree depth (root = 1)
+    float sceneLo[3] = {0, 0, 0}, sceneHi[3] = {0, 0, 0};
+};
+
+// vertices: [n][3][3] floats.  Uses up to `threads` host threads (0 = all).
+Bvh2 buildBvh2(const float* vertices, int n, int threads = 0);
+// Collapse to width 4 or 8 and pack for the device; boxes are padded
+// outwards by `pad` (absolute, in scene units) so the slab test is
+// conservative with respect to the Moller-Trumbore acceptance region.
+WideBvh collapseWide(const Bvh2& b, const float* vertices, int n, int width);
+
+}  // namespace eleven

@@ -1,0 +1,780 @@
+// C-ABI device context: scene upload (renderSetup), rendering (renderCuda),
+// readback (getBuffers / getSamples) and the ray-query hooks.
+//
+// Reference host side (src/tfg-pathtracer/kernel.cu:485-724) deep-copies the
+// AoS scene with one cudaMemcpy per array plus "pointer binding", keeps the
+// accumulation in fixed 1920x1080 device globals and reads it back on a
+// fresh, unsynchronised stream.  Here one context owns one stream; the scene
+// becomes a handful of SoA buffers; accumulation is sized to the camera; and
+// every readback is ordered after the last render on the same stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/eleven.h"
+#include "bvh.hpp"
+#include "hdri_table.hpp"
+#include "kernels.hpp"
+#include "pt_device.hpp"
+
+using namespace eleven;
+using eleven::dev::DevScene;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n && p) return hipSuccess;
+        release();
+        if (bytes == 0) bytes = 16;
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e == hipSuccess) n = bytes;
+        return e;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+constexpr int kCounterSlots = 8;   // [0] rays [1] vertices [2] paths [3] overflow [4] nodes [5] tris
+
+}  // namespace
+
+struct el_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int cus = 0;
+    uint32_t maxThreads = 0;
+    uint32_t block = 256;
+
+    bool hasScene = false;
+    uint32_t ntri = 0;
+    std::vector<float> verts;
+    Bvh2 bvh2;
+    WideBvh wide;
+    int width = 4;
+    int gcap = 1;
+
+    DevScene ds{};
+    el_camera cam{};
+    std::vector<dev::DevMaterial> mats;
+    std::vector<dev::DevTex> texs;
+
+    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvProb, dEnvAlias;
+    DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
+    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wAov, wCounters;
+    size_t wfCap = 0;
+    bool wfAov = false;
+
+    DevBuf tRays, tTmax, tTri, tT, tUV, tOcc;   // scratch for host-pointer ray queries
+
+    uint32_t sampleBase = 0;
+    uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
+    el_stats stats{};
+    bool aovOn = false;
+
+    std::vector<hipEvent_t> evPool;
+};
+
+static int fail(el_ctx* c, int code, const std::string& what) {
+    if (c) c->err = what;
+    return code;
+}
+static int failHip(el_ctx* c, const char* what, hipError_t e) {
+    return fail(c, EL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define EL_HIP(ctx, x)                                   \
+    do {                                                 \
+        hipError_t e_ = (x);                             \
+        if (e_ != hipSuccess) return failHip(ctx, #x, e_); \
+    } while (0)
+
+static hipError_t upload(DevBuf& b, const void* src, size_t bytes, hipStream_t st) {
+    hipError_t e = b.ensure(bytes);
+    if (e != hipSuccess) return e;
+    if (bytes) e = hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st);
+    return e;
+}
+
+static LaunchCfg cfgOf(el_ctx* c) {
+    LaunchCfg l;
+    l.stream = c->stream;
+    l.block = c->block;
+    l.maxThreads = c->maxThreads;
+    l.gstack = c->dGstack.as<int>();
+    return l;
+}
+
+extern "C" {
+
+int el_device_count(int* out) {
+    if (!out) return EL_ERR_ARG;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    *out = n;
+    return EL_OK;
+}
+
+int el_create(el_ctx** out, int device) {
+    if (!out) return EL_ERR_ARG;
+    *out = nullptr;
+    std::unique_ptr<el_ctx> c(new el_ctx());
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return EL_ERR_HIP;
+    if (device < 0 || device >= n) return EL_ERR_ARG;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return EL_ERR_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return EL_ERR_HIP;
+    c->cus = prop.multiProcessorCount;
+    // resident-thread capacity: 32 waves of 64 lanes per CU
+    c->maxThreads = (uint32_t)c->cus * 2048u;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
+    if (c->dCounters.ensure(sizeof(unsigned long long) * kCounterSlots) != hipSuccess) return EL_ERR_HIP;
+    *out = c.release();
+    return EL_OK;
+}
+
+void el_destroy(el_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->evPool) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* el_last_error(const el_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+}  // extern "C"
+
+// Pack and upload the wide BVH of the current width.
+static int uploadBvh(el_ctx* c, int width) {
+    try {
+        c->wide = collapseWide(c->bvh2, c->verts.data(), (int)c->ntri, width);
+    } catch (const std::exception& ex) {
+        return fail(c, EL_ERR_RANGE, ex.what());
+    }
+    c->width = width;
+    EL_HIP(c, upload(c->dNodes, c->wide.nodes.data(), c->wide.nodes.size(), c->stream));
+    EL_HIP(c, upload(c->dTris, c->wide.triStream.data(), c->wide.triStream.size() * sizeof(float), c->stream));
+    c->gcap = std::max(1, c->wide.stackBound + 2 - dev::kShortStack);
+    EL_HIP(c, c->dGstack.ensure((size_t)c->gcap * c->maxThreads * sizeof(int)));
+    c->ds.nodes = c->dNodes.p;
+    c->ds.tris = c->dTris.as<float4>();
+    c->ds.width = width;
+    c->ds.gcap = c->gcap;
+    c->stats.bvh_width = (uint32_t)width;
+    c->stats.stack_bound = (uint32_t)c->wide.stackBound;
+    c->stats.node_bytes = c->wide.nodes.size();
+    c->stats.tri_bytes = c->wide.triStream.size() * sizeof(float);
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    return EL_OK;
+}
+
+static int clearAccum(el_ctx* c) {
+    size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+    EL_HIP(c, hipMemsetAsync(c->dAccum.p, 0, npx * sizeof(float4), c->stream));
+    EL_HIP(c, hipMemsetAsync(c->dCounts.p, 0, npx * sizeof(uint32_t), c->stream));
+    if (c->dAov.p) EL_HIP(c, hipMemsetAsync(c->dAov.p, 0, c->dAov.n, c->stream));
+    c->sampleBase = 0;
+    c->totalRays = c->totalVerts = c->totalPaths = 0;
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    return EL_OK;
+}
+
+extern "C" {
+
+int el_upload_scene(el_ctx* c, const el_scene_view* v) {
+    if (!c || !v) return fail(c, EL_ERR_ARG, "null argument");
+    if (v->tri_count && (!v->vertices || !v->object_id)) return fail(c, EL_ERR_ARG, "missing triangle arrays");
+    if (v->camera.x_res == 0 || v->camera.y_res == 0) return fail(c, EL_ERR_ARG, "camera resolution is zero");
+    if ((uint64_t)v->camera.x_res * v->camera.y_res > (1ull << 28))
+        return fail(c, EL_ERR_RANGE, "resolution above 2^28 pixels");
+    if (v->tri_count >= (uint32_t)kLeafFirstLimit) return fail(c, EL_ERR_RANGE, "more than 2^27 triangles");
+    if (v->material_count == 0 || !v->materials) return fail(c, EL_ERR_ARG, "at least one material is required");
+    if (v->mesh_count == 0 || !v->mesh_material) return fail(c, EL_ERR_ARG, "mesh table missing");
+    if (!v->hdri.data || v->hdri.width <= 0 || v->hdri.height <= 0) return fail(c, EL_ERR_ARG, "hdri missing");
+    if (v->light_count && !v->lights) return fail(c, EL_ERR_ARG, "lights missing");
+    for (uint32_t i = 0; i < v->texture_count; i++)
+        if (!v->textures[i].data || v->textures[i].width <= 0 || v->textures[i].height <= 0)
+            return fail(c, EL_ERR_ARG, "texture " + std::to_string(i) + " is empty");
+    EL_HIP(c, hipSetDevice(c->device));
+    c->hasScene = false;
+    c->ntri = v->tri_count;
+    c->cam = v->camera;
+
+    // geometry + BVH
+    c->verts.assign(v->vertices, v->vertices + (size_t)v->tri_count * 9);
+    c->bvh2 = buildBvh2(c->verts.data(), (int)c->ntri);
+    int rc = uploadBvh(c, c->width);
+    if (rc) return rc;
+
+    // shading stream, indexed by original triangle
+    std::vector<float> shade((size_t)v->tri_count * dev::kShadeStride, 0.f);
+    for (uint32_t i = 0; i < v->tri_count; i++) {
+        float* S = &shade[(size_t)i * dev::kShadeStride];
+        std::memcpy(S, v->vertices + (size_t)i * 9, 9 * sizeof(float));
+        if (v->normals) std::memcpy(S + 9, v->normals + (size_t)i * 9, 9 * sizeof(float));
+        if (v->uvs) std::memcpy(S + 18, v->uvs + (size_t)i * 6, 6 * sizeof(float));
+        if (v->tangents) std::memcpy(S + 24, v->tangents + (size_t)i * 9, 9 * sizeof(float));
+        S[33] = v->tangent_sign ? v->tangent_sign[i] : 0.f;
+        int32_t obj = v->object_id[i];
+        int32_t mat = (obj >= 0 && (uint32_t)obj < v->mesh_count) ? v->mesh_material[obj] : 0;
+        if (mat < 0 || (uint32_t)mat >= v->material_count) mat = 0;
+        std::memcpy(&S[34], &mat, 4);
+    }
+    EL_HIP(c, upload(c->dShade, shade.data(), shade.size() * sizeof(float), c->stream));
+
+    // textures: one pool, the environment map last
+    std::vector<float> pool;
+    c->texs.clear();
+    auto addTex = [&](const el_texture& t) {
+        dev::DevTex d{};
+        d.w = t.width;
+        d.h = t.height;
+        d.xTile = t.x_tile;
+        d.yTile = t.y_tile;
+        d.xOffset = t.x_offset;
+        d.yOffset = t.y_offset;
+        d.filter = t.filter;
+        d.offset = pool.size();
+        pool.insert(pool.end(), t.data, t.data + (size_t)t.width * t.height * 3);
+        return d;
+    };
+    for (uint32_t i = 0; i < v->texture_count; i++) c->texs.push_back(addTex(v->textures[i]));
+    dev::DevTex env = addTex(v->hdri);
+    EL_HIP(c, upload(c->dTexels, pool.data(), pool.size() * sizeof(float), c->stream));
+    EL_HIP(c, upload(c->dTexs, c->texs.data(), c->texs.size() * sizeof(dev::DevTex), c->stream));
+
+    // materials
+    c->mats.assign(v->material_count, dev::DevMaterial{});
+    for (uint32_t i = 0; i < v->material_count; i++) {
+        const el_material& m = v->materials[i];
+        dev::DevMaterial& d = c->mats[i];
+        auto texOk = [&](int32_t id) { return (id >= 0 && (uint32_t)id < v->texture_count) ? id : -1; };
+        d.albedoTex = texOk(m.albedo_tex);
+        d.emissionTex = texOk(m.emission_tex);
+        d.roughnessTex = texOk(m.roughness_tex);
+        d.metallicTex = texOk(m.metallic_tex);
+        d.normalTex = texOk(m.normal_tex);
+        d.albedo[0] = m.albedo.x;
+        d.albedo[1] = m.albedo.y;
+        d.albedo[2] = m.albedo.z;
+        d.emission[0] = m.emission.x;
+        d.emission[1] = m.emission.y;
+        d.emission[2] = m.emission.z;
+        d.roughness = m.roughness;
+        d.metallic = m.metallic;
+        d.clearcoatGloss = m.clearcoat_gloss;
+        d.clearcoat = m.clearcoat;
+        d.anisotropic = m.anisotropic;
+        d.eta = m.eta;
+        d.transmission = m.transmission;
+        d.specular = m.specular;
+        d.specularTint = m.specular_tint;
+        d.sheenTint = m.sheen_tint;
+        d.subsurface = m.subsurface;
+        d.sheen = m.sheen;
+    }
+    EL_HIP(c, upload(c->dMats, c->mats.data(), c->mats.size() * sizeof(dev::DevMaterial), c->stream));
+
+    // lights
+    std::vector<float> lights((size_t)v->light_count * 6);
+    for (uint32_t i = 0; i < v->light_count; i++) {
+        const el_point_light& l = v->lights[i];
+        float* L = &lights[(size_t)i * 6];
+        L[0] = l.position.x;
+        L[1] = l.position.y;
+        L[2] = l.position.z;
+        L[3] = l.radiance.x;
+        L[4] = l.radiance.y;
+        L[5] = l.radiance.z;
+    }
+    EL_HIP(c, upload(c->dLights, lights.data(), lights.size() * sizeof(float), c->stream));
+
+    // HDRI CDF exactly as HDRI::generateCDF at construction (offsets still 0),
+    // then the alias table of the search's effective distribution.
+    const el_texture& h = v->hdri;
+    size_t L = (size_t)h.width * h.height;
+    std::vector<float> cdf(L + 1, 0.f);
+    float sum = 0;
+    for (size_t i = 0; i < L; i++) sum += h.data[3 * i] + h.data[3 * i + 1] + h.data[3 * i + 2];
+    for (size_t i = 0; i < L; i++) cdf[i + 1] = cdf[i] + (h.data[3 * i] + h.data[3 * i + 1] + h.data[3 * i + 2]) / sum;
+    HdriTable tab = buildHdriTable(cdf.data(), (int)L);
+    EL_HIP(c, upload(c->dEnvProb, tab.prob.data(), tab.prob.size() * sizeof(float), c->stream));
+    EL_HIP(c, upload(c->dEnvAlias, tab.alias.data(), tab.alias.size() * sizeof(int32_t), c->stream));
+
+    // accumulation
+    size_t npx = (size_t)v->camera.x_res * v->camera.y_res;
+    EL_HIP(c, c->dAccum.ensure(npx * sizeof(float4)));
+    EL_HIP(c, c->dCounts.ensure(npx * sizeof(uint32_t)));
+    c->dAov.release();
+
+    DevScene& s = c->ds;
+    s.shade = c->dShade.as<float>();
+    s.mats = c->dMats.as<dev::DevMaterial>();
+    s.texs = c->dTexs.as<dev::DevTex>();
+    s.texels = c->dTexels.as<float>();
+    s.lights = c->dLights.as<float>();
+    s.envProb = c->dEnvProb.as<float>();
+    s.envAlias = c->dEnvAlias.as<int32_t>();
+    s.env = env;
+    s.envRadianceSum = sum;
+    s.envLen = (int32_t)L;
+    s.nLights = (int32_t)v->light_count;
+    s.nTris = (int32_t)v->tri_count;
+    dev::DevCamera& dc = s.cam;
+    dc.xRes = v->camera.x_res;
+    dc.yRes = v->camera.y_res;
+    dc.focalLength = v->camera.focal_length;
+    dc.sensorWidth = v->camera.sensor_width;
+    dc.sensorHeight = v->camera.sensor_height;
+    dc.aperture = v->camera.aperture;
+    dc.focusDistance = v->camera.focus_distance;
+    dc.rot[0] = v->camera.rotation.x;
+    dc.rot[1] = v->camera.rotation.y;
+    dc.rot[2] = v->camera.rotation.z;
+    dc.pos[0] = v->camera.position.x;
+    dc.pos[1] = v->camera.position.y;
+    dc.pos[2] = v->camera.position.z;
+    dc.bokeh = v->camera.bokeh;
+    rc = clearAccum(c);
+    if (rc) return rc;
+    c->hasScene = true;
+    return EL_OK;
+}
+
+int el_clear(el_ctx* c) {
+    if (!c) return EL_ERR_ARG;
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    EL_HIP(c, hipSetDevice(c->device));
+    return clearAccum(c);
+}
+
+}  // extern "C"
+
+static hipEvent_t poolEvent(el_ctx* c, size_t i) {
+    while (c->evPool.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->evPool.push_back(e);
+    }
+    return c->evPool[i];
+}
+
+static int ensureWf(el_ctx* c, size_t cap, bool aov) {
+    if (cap <= c->wfCap && (!aov || c->wfAov)) return EL_OK;
+    EL_HIP(c, c->wRay0.ensure(cap * 32));
+    EL_HIP(c, c->wRay1.ensure(cap * 32));
+    EL_HIP(c, c->wHits.ensure(cap * 16));
+    EL_HIP(c, c->wShH.ensure(cap * 48));
+    EL_HIP(c, c->wShP.ensure(cap * 48));
+    EL_HIP(c, c->wT.ensure(cap * 16));
+    EL_HIP(c, c->wL.ensure(cap * 16));
+    EL_HIP(c, c->wPix.ensure(cap * 4));
+    EL_HIP(c, c->wSample.ensure(cap * 4));
+    if (aov) EL_HIP(c, c->wAov.ensure(cap * 48));
+    EL_HIP(c, c->wCounters.ensure(64));
+    c->wfCap = cap;
+    c->wfAov = aov;
+    return EL_OK;
+}
+
+extern "C" int el_render(el_ctx* c, const el_render_params* p) {
+    if (!c || !p) return fail(c, EL_ERR_ARG, "null argument");
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (p->mode != EL_MODE_PLAIN && p->mode != EL_MODE_WAVEFRONT) return fail(c, EL_ERR_ARG, "unknown mode");
+    if (p->bvh_width != 0 && p->bvh_width != 4 && p->bvh_width != 8) return fail(c, EL_ERR_ARG, "bvh_width must be 4 or 8");
+    if (p->block != 0 && (p->block % 64 != 0 || p->block > 1024)) return fail(c, EL_ERR_ARG, "block must be a multiple of 64, <= 1024");
+    uint32_t depth = p->max_depth ? p->max_depth : 5;
+    if (depth > 64) return fail(c, EL_ERR_RANGE, "max_depth above 64");
+    EL_HIP(c, hipSetDevice(c->device));
+    if (p->spp == 0) return EL_OK;
+    if (p->bvh_width && (int)p->bvh_width != c->width) {
+        int rc = uploadBvh(c, (int)p->bvh_width);
+        if (rc) return rc;
+    }
+    c->block = p->block ? p->block : 256;
+    bool aov = (p->flags & EL_FLAG_AOV) != 0;
+    size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+    if (aov && !c->dAov.p) {
+        EL_HIP(c, c->dAov.ensure(npx * 3 * sizeof(float4)));
+        EL_HIP(c, hipMemsetAsync(c->dAov.p, 0, c->dAov.n, c->stream));
+    }
+    c->aovOn = c->aovOn || aov;
+    float4* aovBuf = aov ? c->dAov.as<float4>() : nullptr;
+    unsigned long long* ctr = c->dCounters.as<unsigned long long>();
+    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterSlots, c->stream));
+    LaunchCfg lc = cfgOf(c);
+    RenderArgs a{};
+    a.k0 = (uint32_t)p->seed;
+    a.k1 = (uint32_t)(p->seed >> 32);
+    a.scene = 0;
+    a.maxDepth = depth;
+    a.sample = c->sampleBase;
+
+    hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
+    if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
+    double extendMs = 0, shadeMs = 0, shadowMs = 0;
+    uint64_t extendLaunches = 0;
+    EL_HIP(c, hipEventRecord(e0, c->stream));
+    const bool countTrav = (p->flags & 2u) != 0;
+
+    if (p->mode == EL_MODE_PLAIN) {
+        for (uint32_t s = 0; s < p->spp; s++) {
+            a.sample = c->sampleBase + s;
+            EL_HIP(c, launchPlain(lc, c->ds, a, c->dAccum.as<float4>(), c->dCounts.as<uint32_t>(), aovBuf, ctr));
+        }
+    } else {
+        const uint32_t tilesX = (c->cam.x_res + 7) / 8, tilesY = (c->cam.y_res + 7) / 8;
+        const uint64_t npad = (uint64_t)tilesX * tilesY * 64;
+        uint64_t total = (uint64_t)p->spp * npad;
+        uint64_t Q = p->queue_cap ? p->queue_cap : (1u << 22);
+        if (Q >= npad)
+            Q = (Q / npad) * npad;
+        else
+            Q = std::max<uint64_t>(64, (Q / 64) * 64);
+        if (Q > total) Q = total;
+        if (Q > 0x7fffffffull) return fail(c, EL_ERR_RANGE, "queue capacity above 2^31");
+        int rc = ensureWf(c, Q, aov);
+        if (rc) return rc;
+        WfState q{};
+        q.ray[0] = c->wRay0.as<float4>();
+        q.ray[1] = c->wRay1.as<float4>();
+        q.hits = c->wHits.as<float4>();
+        q.shadowH = c->wShH.as<float4>();
+        q.shadowP = c->wShP.as<float4>();
+        q.pathT = c->wT.as<float4>();
+        q.pathL = c->wL.as<float4>();
+        q.pathPix = c->wPix.as<uint32_t>();
+        q.pathSample = c->wSample.as<uint32_t>();
+        q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
+        q.counters = c->wCounters.as<uint32_t>();
+        q.gstack = lc.gstack;
+        q.gthreads = lc.maxThreads;
+        a.npad = (uint32_t)npad;
+        size_t ev = 2;
+        std::vector<std::pair<size_t, int>> timed;   // (event index, kind) kind 0 extend 1 shade 2 shadow
+        for (uint64_t start = 0; start < total; start += Q) {
+            a.genStart = start;
+            a.genCount = (uint32_t)std::min<uint64_t>(Q, total - start);
+            EL_HIP(c, launchFillU32(c->stream, q.counters, 0, 4));
+            EL_HIP(c, launchWfCamera(lc, c->ds, a, q));
+            for (uint32_t b = 0; b < depth; b++) {
+                a.bounce = b;
+                int cur = b & 1, nxt = cur ^ 1;
+                hipEvent_t x0 = poolEvent(c, ev), x1 = poolEvent(c, ev + 1);
+                EL_HIP(c, hipEventRecord(x0, c->stream));
+                EL_HIP(c, launchWfExtend(lc, c->ds, q, q.ray[cur], q.counters + cur, nxt, a.genCount, countTrav,
+                                         ctr));
+                EL_HIP(c, hipEventRecord(x1, c->stream));
+                timed.push_back({ev, 0});
+                ev += 2;
+                hipEvent_t y0 = poolEvent(c, ev), y1 = poolEvent(c, ev + 1);
+                EL_HIP(c, hipEventRecord(y0, c->stream));
+                EL_HIP(c, launchWfShade(lc, c->ds, a, q, q.ray[cur], q.counters + cur, q.ray[nxt],
+                                        q.counters + nxt, a.genCount, ctr));
+                EL_HIP(c, hipEventRecord(y1, c->stream));
+                timed.push_back({ev, 1});
+                ev += 2;
+                hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
+                EL_HIP(c, hipEventRecord(z0, c->stream));
+                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowH, q.counters + 2, a.genCount, ctr));
+                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowP, q.counters + 3, a.genCount, ctr));
+                EL_HIP(c, hipEventRecord(z1, c->stream));
+                timed.push_back({ev, 2});
+                ev += 2;
+                extendLaunches++;
+            }
+            EL_HIP(c, launchWfFinalize(lc, c->ds, a, q, c->dAccum.as<float4>(), c->dCounts.as<uint32_t>(), aovBuf,
+                                       ctr));
+        }
+        EL_HIP(c, hipEventRecord(e1, c->stream));
+        EL_HIP(c, hipEventSynchronize(e1));
+        for (auto& t : timed) {
+            float ms = 0;
+            EL_HIP(c, hipEventElapsedTime(&ms, c->evPool[t.first], c->evPool[t.first + 1]));
+            (t.second == 0 ? extendMs : t.second == 1 ? shadeMs : shadowMs) += ms;
+        }
+    }
+    if (p->mode == EL_MODE_PLAIN) {
+        EL_HIP(c, hipEventRecord(e1, c->stream));
+        EL_HIP(c, hipEventSynchronize(e1));
+    }
+    float totalMs = 0;
+    EL_HIP(c, hipEventElapsedTime(&totalMs, e0, e1));
+    unsigned long long h[kCounterSlots];
+    EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    c->sampleBase += p->spp;
+    c->totalRays += h[0];
+    c->totalVerts += h[1];
+    c->totalPaths += h[2];
+    c->stats.rays = h[0];
+    c->stats.vertices = h[1];
+    c->stats.paths = h[2];
+    c->stats.render_ms = totalMs;
+    c->stats.extend_ms = extendMs;
+    c->stats.shade_ms = shadeMs;
+    c->stats.shadow_ms = shadowMs;
+    c->stats.extend_launches = extendLaunches;
+    c->stats.extend_rays = 0;
+    c->stats.node_fetches = h[4];
+    c->stats.tri_tests = h[5];
+    if (h[3]) return fail(c, EL_ERR_RANGE, "traversal stack overflow (results incomplete)");
+    return EL_OK;
+}
+
+extern "C" {
+
+int el_read_passes(el_ctx* c, float* rgba, uint32_t* samples, uint64_t* rays, uint64_t* vertices) {
+    if (!c) return EL_ERR_ARG;
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    EL_HIP(c, hipSetDevice(c->device));
+    size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+    std::vector<float> acc(npx * 4);
+    std::vector<uint32_t> cnt(npx);
+    EL_HIP(c, hipMemcpyAsync(acc.data(), c->dAccum.p, npx * sizeof(float4), hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipMemcpyAsync(cnt.data(), c->dCounts.p, npx * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    if (rgba)
+        for (size_t i = 0; i < npx; i++) {
+            float n = (float)cnt[i];
+            for (int k = 0; k < 3; k++) rgba[4 * i + k] = cnt[i] ? acc[4 * i + k] / n : 0.f;
+            rgba[4 * i + 3] = 1.f;
+        }
+    if (samples) std::memcpy(samples, cnt.data(), npx * sizeof(uint32_t));
+    if (rays) *rays = c->totalRays;
+    if (vertices) *vertices = c->totalVerts;
+    return EL_OK;
+}
+
+int el_read_aov(el_ctx* c, int pass, float* rgba) {
+    if (!c || !rgba) return EL_ERR_ARG;
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (pass != EL_PASS_NORMAL && pass != EL_PASS_TANGENT && pass != EL_PASS_BITANGENT)
+        return fail(c, EL_ERR_ARG, "pass must be NORMAL, TANGENT or BITANGENT");
+    if (!c->dAov.p) return fail(c, EL_ERR_STATE, "no AOV rendered (EL_FLAG_AOV)");
+    EL_HIP(c, hipSetDevice(c->device));
+    size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+    std::vector<float> a(npx * 12);
+    std::vector<uint32_t> cnt(npx);
+    EL_HIP(c, hipMemcpyAsync(a.data(), c->dAov.p, npx * 48, hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipMemcpyAsync(cnt.data(), c->dCounts.p, npx * 4, hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    int k = pass - EL_PASS_NORMAL;
+    for (size_t i = 0; i < npx; i++) {
+        for (int j = 0; j < 3; j++) rgba[4 * i + j] = cnt[i] ? a[12 * i + 4 * k + j] / (float)cnt[i] : 0.f;
+        rgba[4 * i + 3] = 1.f;
+    }
+    return EL_OK;
+}
+
+int el_get_stats(el_ctx* c, el_stats* out) {
+    if (!c || !out) return EL_ERR_ARG;
+    *out = c->stats;
+    return EL_OK;
+}
+
+int el_set_materials(el_ctx* c, const el_material* m, uint32_t count) {
+    if (!c || !m) return EL_ERR_ARG;
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (count != c->mats.size()) return fail(c, EL_ERR_ARG, "material count differs from the uploaded scene");
+    for (uint32_t i = 0; i < count; i++) {
+        dev::DevMaterial& d = c->mats[i];
+        d.albedo[0] = m[i].albedo.x;
+        d.albedo[1] = m[i].albedo.y;
+        d.albedo[2] = m[i].albedo.z;
+        d.emission[0] = m[i].emission.x;
+        d.emission[1] = m[i].emission.y;
+        d.emission[2] = m[i].emission.z;
+        d.roughness = m[i].roughness;
+        d.metallic = m[i].metallic;
+        d.clearcoatGloss = m[i].clearcoat_gloss;
+        d.clearcoat = m[i].clearcoat;
+        d.anisotropic = m[i].anisotropic;
+        d.eta = m[i].eta;
+        d.transmission = m[i].transmission;
+        d.specular = m[i].specular;
+        d.specularTint = m[i].specular_tint;
+        d.sheenTint = m[i].sheen_tint;
+        d.subsurface = m[i].subsurface;
+        d.sheen = m[i].sheen;
+    }
+    EL_HIP(c, hipSetDevice(c->device));
+    EL_HIP(c, hipMemcpyAsync(c->dMats.p, c->mats.data(), count * sizeof(dev::DevMaterial), hipMemcpyHostToDevice,
+                             c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    return EL_OK;
+}
+
+int el_set_texture(el_ctx* c, int32_t id, const float* rgb) {
+    if (!c || !rgb) return EL_ERR_ARG;
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (id < 0 || (size_t)id >= c->texs.size()) return fail(c, EL_ERR_ARG, "texture id out of range");
+    const dev::DevTex& t = c->texs[id];
+    EL_HIP(c, hipSetDevice(c->device));
+    EL_HIP(c, hipMemcpyAsync(c->dTexels.as<float>() + t.offset, rgb, (size_t)t.w * t.h * 3 * sizeof(float),
+                             hipMemcpyHostToDevice, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    return EL_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- queries --
+static int traceCommon(el_ctx* c, const el_ray* rays, const float* tmax, uint32_t n, int32_t* tri, float* t,
+                       float* uv, uint8_t* occ, int devp, bool any, bool count, uint64_t* nodes, uint64_t* tris) {
+    if (!c || !rays) return fail(c, EL_ERR_ARG, "null argument");
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (any && (!tmax || !occ)) return fail(c, EL_ERR_ARG, "null argument");
+    if (!any && !count && (!tri || !t)) return fail(c, EL_ERR_ARG, "null argument");
+    if (n == 0) return EL_OK;
+    EL_HIP(c, hipSetDevice(c->device));
+    const float* dR = (const float*)rays;
+    const float* dTm = tmax;
+    int32_t* dTri = tri;
+    float* dT = t;
+    float* dUV = uv;
+    uint8_t* dOcc = occ;
+    if (!devp) {
+        EL_HIP(c, upload(c->tRays, rays, (size_t)n * sizeof(el_ray), c->stream));
+        dR = c->tRays.as<float>();
+        if (any) {
+            EL_HIP(c, upload(c->tTmax, tmax, (size_t)n * 4, c->stream));
+            dTm = c->tTmax.as<float>();
+            EL_HIP(c, c->tOcc.ensure(n));
+            dOcc = c->tOcc.as<uint8_t>();
+        } else {
+            EL_HIP(c, c->tTri.ensure((size_t)n * 4));
+            EL_HIP(c, c->tT.ensure((size_t)n * 4));
+            dTri = c->tTri.as<int32_t>();
+            dT = c->tT.as<float>();
+            if (uv) {
+                EL_HIP(c, c->tUV.ensure((size_t)n * 8));
+                dUV = c->tUV.as<float>();
+            }
+        }
+    } else if (count) {
+        EL_HIP(c, c->tTri.ensure((size_t)n * 4));
+        EL_HIP(c, c->tT.ensure((size_t)n * 4));
+        if (!dTri) dTri = c->tTri.as<int32_t>();
+        if (!dT) dT = c->tT.as<float>();
+    }
+    if (!devp && count) {
+        EL_HIP(c, c->tTri.ensure((size_t)n * 4));
+        EL_HIP(c, c->tT.ensure((size_t)n * 4));
+        dTri = c->tTri.as<int32_t>();
+        dT = c->tT.as<float>();
+    }
+    unsigned long long* ctr = c->dCounters.as<unsigned long long>();
+    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterSlots, c->stream));
+    EL_HIP(c, launchTrace(cfgOf(c), c->ds, dR, dTm, n, dTri, dT, dUV, dOcc, any, count, ctr));
+    unsigned long long h[kCounterSlots];
+    EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    if (!devp) {
+        if (any) {
+            EL_HIP(c, hipMemcpyAsync(occ, dOcc, n, hipMemcpyDeviceToHost, c->stream));
+        } else if (!count) {
+            EL_HIP(c, hipMemcpyAsync(tri, dTri, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+            EL_HIP(c, hipMemcpyAsync(t, dT, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
+            if (uv) EL_HIP(c, hipMemcpyAsync(uv, dUV, (size_t)n * 8, hipMemcpyDeviceToHost, c->stream));
+        }
+    }
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    if (nodes) *nodes = h[0];
+    if (tris) *tris = h[1];
+    if (h[2]) return fail(c, EL_ERR_RANGE, "traversal stack overflow (results incomplete)");
+    return EL_OK;
+}
+
+extern "C" {
+
+int el_trace_closest(el_ctx* c, const el_ray* rays, uint32_t n, int32_t* tri, float* t, float* uv, int devp) {
+    return traceCommon(c, rays, nullptr, n, tri, t, uv, nullptr, devp, false, false, nullptr, nullptr);
+}
+
+int el_trace_any(el_ctx* c, const el_ray* rays, const float* tmax, uint32_t n, uint8_t* occ, int devp) {
+    return traceCommon(c, rays, tmax, n, nullptr, nullptr, nullptr, occ, devp, true, false, nullptr, nullptr);
+}
+
+int el_trace_count(el_ctx* c, const el_ray* rays, uint32_t n, uint64_t* nodes, uint64_t* tris, int devp) {
+    if (!nodes || !tris) return fail(c, EL_ERR_ARG, "null argument");
+    return traceCommon(c, rays, nullptr, n, nullptr, nullptr, nullptr, nullptr, devp, false, true, nodes, tris);
+}
+
+int el_trace_bench(el_ctx* c, const el_ray* d_rays, uint32_t n, int32_t* d_tri, float* d_t, int iters,
+                   double* mean_ms) {
+    if (!c || !d_rays || !d_tri || !d_t || !mean_ms || iters <= 0) return fail(c, EL_ERR_ARG, "null argument");
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    EL_HIP(c, hipSetDevice(c->device));
+    unsigned long long* ctr = c->dCounters.as<unsigned long long>();
+    hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
+    double total = 0;
+    for (int i = 0; i < iters; i++) {
+        EL_HIP(c, hipEventRecord(e0, c->stream));
+        EL_HIP(c, launchTrace(cfgOf(c), c->ds, (const float*)d_rays, nullptr, n, d_tri, d_t, nullptr, nullptr, false,
+                              false, ctr));
+        EL_HIP(c, hipEventRecord(e1, c->stream));
+        EL_HIP(c, hipEventSynchronize(e1));
+        float ms = 0;
+        EL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+        total += ms;
+    }
+    *mean_ms = total / iters;
+    return EL_OK;
+}
+
+int el_device_alloc(el_ctx* c, size_t bytes, void** out) {
+    if (!c || !out) return EL_ERR_ARG;
+    EL_HIP(c, hipSetDevice(c->device));
+    EL_HIP(c, hipMalloc(out, bytes ? bytes : 16));
+    return EL_OK;
+}
+
+int el_device_free(el_ctx* c, void* p) {
+    if (!c) return EL_ERR_ARG;
+    EL_HIP(c, hipSetDevice(c->device));
+    EL_HIP(c, hipFree(p));
+    return EL_OK;
+}
+
+int el_memcpy_htod(el_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) return EL_ERR_ARG;
+    EL_HIP(c, hipSetDevice(c->device));
+    EL_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    return EL_OK;
+}
+
+int el_memcpy_dtoh(el_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!c || !dst || !src) return EL_ERR_ARG;
+    EL_HIP(c, hipSetDevice(c->device));
+    EL_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    return EL_OK;
+}
+
+}  // extern "C"

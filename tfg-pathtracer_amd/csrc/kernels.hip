@@ -1,0 +1,565 @@
+// HIP kernels of the path-tracing core for gfx950 (MI355X, wave64).
+//
+//   k_trace        closest / any hit for a ray batch (parity + traversal bench)
+//   k_plain        one thread per pixel, full path per thread: the plain
+//                  megakernel, same structure as renderingKernel
+//                  (kernel.cu:369-481) with per-sample launches
+//                  (kernel.cu:675-683)
+//   k_wf_*         wavefront: camera -> extend -> shade -> shadow x2 ->
+//                  finalize, with per-bounce ray queues compacted by wave
+//                  ballot + one atomic per wave
+//
+// All traversal kernels are grid-stride loops over a grid no larger than the
+// device's resident-thread capacity, so the per-thread global spill area of
+// the traversal stack is sized once per context.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "pt_device.hpp"
+
+namespace eleven {
+namespace dev {
+
+__device__ __forceinline__ uint32_t waveSumU32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ void flushCount(unsigned long long* dst, uint32_t v) {
+    uint32_t s = waveSumU32(v);
+    if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+}
+
+// Wave-wide slot allocation: one atomic per wave, lanes get consecutive slots.
+__device__ __forceinline__ int waveAlloc(uint32_t* counter, bool want) {
+    unsigned long long mask = __ballot(want);
+    if (mask == 0) return -1;
+    int lane = __lane_id();
+    int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (!want) return -1;
+    return (int)(base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+__device__ __forceinline__ TravStack makeStack(const DevScene& s, int* gstack, uint32_t gthreads) {
+    extern __shared__ int lds[];
+    TravStack st;
+    st.lds = lds + threadIdx.x;
+    st.ldsStride = blockDim.x;
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    st.g = gstack + gtid;
+    st.gstride = (int)gthreads;
+    st.gcap = s.gcap;
+    st.sp = st.lo = 0;
+    st.overflow = false;
+    return st;
+}
+
+// ------------------------------------------------------------------ trace --
+template <int W, bool ANY, bool COUNT>
+__global__ void k_trace(DevScene s, const float* __restrict__ rays, const float* __restrict__ tmax, uint32_t n,
+                        int32_t* __restrict__ outTri, float* __restrict__ outT, float* __restrict__ outUV,
+                        uint8_t* __restrict__ outOcc, int* gstack, uint32_t gthreads,
+                        unsigned long long* counters) {
+    TravStack st = makeStack(s, gstack, gthreads);
+    uint32_t stride = gridDim.x * blockDim.x;
+    TraceCounters cnt{0, 0};
+    uint32_t overflow = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const float* r = rays + (uint64_t)i * 6;
+        float tm = ANY ? tmax[i] : __int_as_float(0x7f800000);
+        TraceResult h = traceRay<W, ANY, COUNT>(s, r[0], r[1], r[2], r[3], r[4], r[5], tm, st, cnt);
+        overflow |= st.overflow;
+        if (ANY) {
+            outOcc[i] = h.tri >= 0 ? 1 : 0;
+        } else {
+            outTri[i] = h.tri;
+            outT[i] = h.tri >= 0 ? h.t : __int_as_float(0x7f800000);
+            if (outUV) {
+                outUV[2 * (uint64_t)i] = h.u;
+                outUV[2 * (uint64_t)i + 1] = h.v;
+            }
+        }
+    }
+    if (COUNT) {
+        flushCount(&counters[0], cnt.nodes);
+        flushCount(&counters[1], cnt.tris);
+    }
+    flushCount(&counters[2], overflow);
+}
+
+// --------------------------------------------------------- shared shading --
+struct VertexOut {
+    v3 base;        // T * (w1*0 + w2*0 + w3*B): added unconditionally
+    v3 contribH;    // T * w1 * H_unoccluded: added if the HDRI shadow ray is clear
+    v3 contribP;    // T * w2 * P_unoccluded: added if the point shadow ray is clear
+    v3 hOrig, hDir; // HDRI shadow ray
+    v3 pOrig, pDir; // point-light shadow ray
+    float pTmax;
+    bool hasP;
+    v3 newT;        // throughput after the bounce
+    v3 nextO, nextD;
+};
+
+// One path vertex: DisneySample, DisneyEval/Pdf, HDRI and point-light NEE
+// terms and the balance-heuristic combine of shade() (kernel.cu:339-358),
+// with the shadow queries left to the caller.
+__device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, const SurfaceHit& sh,
+                                                 const HitData& hd, const uint32_t* r) {
+    VertexOut o;
+    v3 bdir = disneySample(d, hd, u01(r[0]), u01(r[1]), u01(r[2]));
+    v3 f = disneyEval(d, hd, bdir);
+    float bpdf = disneyPdf(d, hd, bdir);
+    float r1 = u01(r[3]);
+
+    EnvSample es = envSample(s, r[3], u01(r[4]));
+    v3 H = disneyEval(d, hd, es.dir) * fabsf(dot(es.dir, hd.normal)) * es.value / es.pdf;
+    float hpdf = es.pdf;
+    o.hOrig = sh.position + es.dir * 0.001f;
+    o.hDir = normalized(es.dir);
+
+    float ppdf = 0.f;
+    v3 P = mk(0, 0, 0);
+    o.hasP = s.nLights > 0;
+    if (o.hasP) {
+        ppdf = ((float)s.nLights) / (float)(2.0 * EL_PI);
+        int li = (int)(s.nLights * r1);
+        li = li < s.nLights ? li : s.nLights - 1;
+        v3 lp = ld3(s.lights + 6 * li), lr = ld3(s.lights + 6 * li + 3);
+        v3 nd = normalized(lp - sh.position);
+        float dist = length(lp - sh.position);
+        o.pOrig = sh.position + nd * 0.001f;
+        o.pDir = normalized(nd);
+        o.pTmax = dist - 0.001f;
+        P = (lr / (dist * dist)) * disneyEval(d, hd, nd) * fabsf(dot(nd, hd.normal)) / ppdf;
+    }
+    v3 fc = f * fabsf(dot(bdir, hd.normal));
+    v3 B = hd.emission * fc / bpdf;
+    float sum = hpdf + ppdf + bpdf;
+    float w1 = hpdf / sum, w2 = ppdf / sum, w3 = bpdf / sum;
+    o.base = T * (w1 * mk(0, 0, 0) + w2 * mk(0, 0, 0) + w3 * B);
+    o.contribH = T * (w1 * H);
+    o.contribP = T * (w2 * P);
+    o.newT = T * (fc / bpdf);
+    o.nextO = sh.position + bdir * 0.001f;
+    o.nextD = normalized(bdir);
+    return o;
+}
+
+__device__ __forceinline__ bool anyNan(v3 a) { return isnan(a.x) || isnan(a.y) || isnan(a.z); }
+__device__ __forceinline__ v3 clamp3(v3 a) {
+    return mk(clampf(a.x, 0.f, 10.f), clampf(a.y, 0.f, 10.f), clampf(a.z, 0.f, 10.f));
+}
+
+// ------------------------------------------------------------------ plain --
+template <int W>
+__global__ void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, uint32_t* __restrict__ counts,
+                        float4* __restrict__ aov, int* gstack, uint32_t gthreads,
+                        unsigned long long* counters) {
+    TravStack st = makeStack(s, gstack, gthreads);
+    TraceCounters cnt{0, 0};
+    const uint32_t W_ = s.cam.xRes, H_ = s.cam.yRes;
+    const uint32_t tilesX = (W_ + 15) / 16;
+    uint32_t rays = 0, verts = 0, paths = 0, overflow = 0;
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t total = tilesX * ((H_ + 15) / 16) * 256;
+    for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < total; w += stride) {
+        uint32_t tile = w >> 8, in = w & 255;
+        uint32_t x = (tile % tilesX) * 16 + (in & 15), y = (tile / tilesX) * 16 + (in >> 4);
+        if (x >= W_ || y >= H_) continue;
+        uint32_t idx = W_ * (H_ - y - 1) + x;
+        PathRng rng{a.k0, a.k1, idx, a.sample, a.scene};
+        uint32_t c[5];
+        rng.dims(0, 5, c);
+        v3 o, d;
+        cameraRay(s.cam, (int)x, (int)y, u01(c[0]), u01(c[1]), u01(c[2]), u01(c[3]), u01(c[4]), o, d);
+        v3 L = mk(0, 0, 0), T = mk(1, 1, 1);
+        v3 an = mk(0, 0, 0), at = mk(0, 0, 0), ab = mk(0, 0, 0);
+        int i;
+        paths++;
+        for (i = 0; i < (int)a.maxDepth; i++) {
+            rays++;
+            TraceResult h = traceRay<W, false, false>(s, o.x, o.y, o.z, d.x, d.y, d.z,
+                                                       __int_as_float(0x7f800000), st, cnt);
+            if (h.tri < 0) {
+                L = L + envLookup(s, d) * T;
+                break;
+            }
+            SurfaceHit sh = surfaceAt(s, h.tri, h.t, h.u, h.v, o, d);
+            HitData hd = hitData(s, sh);
+            uint32_t r[5];
+            rng.dims(5 + 5 * i, 5, r);
+            VertexOut vo = shadeVertex(s, d, T, sh, hd, r);
+            rays++;
+            TraceResult hs = traceRay<W, true, false>(s, vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, vo.hDir.x, vo.hDir.y,
+                                                      vo.hDir.z, __int_as_float(0x7f800000), st, cnt);
+            v3 add = vo.base;
+            if (hs.tri < 0) add = add + vo.contribH;
+            if (vo.hasP) {
+                rays++;
+                TraceResult hp = traceRay<W, true, false>(s, vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, vo.pDir.x,
+                                                          vo.pDir.y, vo.pDir.z, vo.pTmax, st, cnt);
+                if (hp.tri < 0) add = add + vo.contribP;
+            }
+            L = L + add;
+            if (i == 0) {
+                an = sh.normal;
+                at = sh.tangent;
+                ab = sh.bitangent;
+            }
+            T = vo.newT;
+            o = vo.nextO;
+            d = vo.nextD;
+        }
+        overflow |= st.overflow;
+        verts += (uint32_t)i;
+        L = clamp3(L);
+        if (!anyNan(L)) {
+            float4 acc = accum[idx];
+            accum[idx] = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
+            counts[idx] += 1;
+            if (aov) {
+                float4* A = aov + 3 * (uint64_t)idx;
+                A[0] = make_float4(A[0].x + an.x, A[0].y + an.y, A[0].z + an.z, 0);
+                A[1] = make_float4(A[1].x + at.x, A[1].y + at.y, A[1].z + at.z, 0);
+                A[2] = make_float4(A[2].x + ab.x, A[2].y + ab.y, A[2].z + ab.z, 0);
+            }
+        }
+    }
+    flushCount(&counters[0], rays);
+    flushCount(&counters[1], verts);
+    flushCount(&counters[2], paths);
+    flushCount(&counters[3], overflow);
+}
+
+// -------------------------------------------------------------- wavefront --
+// Work item w (0 <= w < spp * npad): sample w / npad, padded pixel w % npad,
+// padded pixels in 8x8 tiles.
+__device__ __forceinline__ bool workPixel(const DevScene& s, uint32_t p, uint32_t& x, uint32_t& y) {
+    uint32_t tilesX = (s.cam.xRes + 7) / 8;
+    uint32_t tile = p >> 6, in = p & 63;
+    x = (tile % tilesX) * 8 + (in & 7);
+    y = (tile / tilesX) * 8 + (in >> 3);
+    return x < s.cam.xRes && y < s.cam.yRes;
+}
+
+__global__ void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t* cnt = q.counters + 0;   // bounce 0 reads queue 0
+    for (uint32_t b = blockIdx.x * blockDim.x; b < a.genCount; b += stride) {
+        uint32_t slot = b + threadIdx.x;
+        bool valid = slot < a.genCount;
+        uint32_t x = 0, y = 0, idx = 0, sample = 0;
+        if (valid) {
+            uint64_t w = a.genStart + slot;
+            uint32_t p = (uint32_t)(w % a.npad);
+            sample = a.sample + (uint32_t)(w / a.npad);
+            valid = workPixel(s, p, x, y);
+            idx = s.cam.xRes * (s.cam.yRes - y - 1) + x;
+        }
+        if (slot < a.genCount) {
+            q.pathT[slot] = make_float4(1, 1, 1, 0);
+            q.pathL[slot] = make_float4(0, 0, 0, valid ? 1.f : 0.f);   // w: path is live
+            q.pathPix[slot] = valid ? idx : 0xFFFFFFFFu;
+            q.pathSample[slot] = sample;
+            if (q.pathAov) {
+                q.pathAov[3 * (uint64_t)slot + 0] = make_float4(0, 0, 0, 0);
+                q.pathAov[3 * (uint64_t)slot + 1] = make_float4(0, 0, 0, 0);
+                q.pathAov[3 * (uint64_t)slot + 2] = make_float4(0, 0, 0, 0);
+            }
+        }
+        int qi = waveAlloc(cnt, valid);
+        if (valid) {
+            PathRng rng{a.k0, a.k1, idx, sample, a.scene};
+            uint32_t c[5];
+            rng.dims(0, 5, c);
+            v3 o, d;
+            cameraRay(s.cam, (int)x, (int)y, u01(c[0]), u01(c[1]), u01(c[2]), u01(c[3]), u01(c[4]), o, d);
+            float4* Q = q.ray[0] + 2 * (uint64_t)qi;
+            Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
+            Q[1] = make_float4(d.x, d.y, d.z, 0.f);
+        }
+    }
+}
+
+// Also clears the counters the following shade kernel appends to: the next
+// ray queue (`resetIdx`) and both shadow queues.
+template <int W>
+__global__ void k_wf_extend(DevScene s, WfState q, const float4* __restrict__ rayIn, const uint32_t* countIn,
+                            int resetIdx, int countTraversal, unsigned long long* counters) {
+    TravStack st = makeStack(s, q.gstack, q.gthreads);
+    TraceCounters cnt{0, 0};
+    uint32_t n = *countIn;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        q.counters[resetIdx] = 0;
+        q.counters[2] = 0;
+        q.counters[3] = 0;
+    }
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t rays = 0, overflow = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float4 A = rayIn[2 * (uint64_t)i], B = rayIn[2 * (uint64_t)i + 1];
+        TraceResult h;
+        if (countTraversal)
+            h = traceRay<W, false, true>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
+        else
+            h = traceRay<W, false, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
+        overflow |= st.overflow;
+        q.hits[i] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
+        rays++;
+    }
+    flushCount(&counters[0], rays);
+    flushCount(&counters[3], overflow);
+    if (countTraversal) {
+        flushCount(&counters[4], cnt.nodes);
+        flushCount(&counters[5], cnt.tris);
+    }
+}
+
+__global__ void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
+                           const uint32_t* countIn, float4* __restrict__ rayOut, uint32_t* countOut,
+                           unsigned long long* counters) {
+    uint32_t n = *countIn;
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t rays = 0, verts = 0;
+    const int bounce = (int)a.bounce;
+    const bool last = bounce + 1 >= (int)a.maxDepth;
+    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += stride) {
+        uint32_t i = b + threadIdx.x;
+        bool valid = i < n;
+        bool cont = false, wantH = false, wantP = false;
+        VertexOut vo;
+        uint32_t slot = 0;
+        if (valid) {
+            float4 A = rayIn[2 * (uint64_t)i], B = rayIn[2 * (uint64_t)i + 1];
+            float4 hr = q.hits[i];
+            slot = __float_as_uint(A.w);
+            v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
+            float4 Tf = q.pathT[slot];
+            v3 T = mk(Tf.x, Tf.y, Tf.z);
+            int tri = __float_as_int(hr.x);
+            if (tri < 0) {
+                v3 e = envLookup(s, d) * T;
+                float4 L = q.pathL[slot];
+                q.pathL[slot] = make_float4(L.x + e.x, L.y + e.y, L.z + e.z, L.w);
+            } else {
+                SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
+                HitData hd = hitData(s, sh);
+                uint32_t pix = q.pathPix[slot];
+                PathRng rng{a.k0, a.k1, pix, q.pathSample[slot], a.scene};
+                uint32_t r[5];
+                rng.dims(5 + 5 * bounce, 5, r);
+                vo = shadeVertex(s, d, T, sh, hd, r);
+                float4 L = q.pathL[slot];
+                q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
+                q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
+                if (bounce == 0 && q.pathAov) {
+                    q.pathAov[3 * (uint64_t)slot + 0] = make_float4(sh.normal.x, sh.normal.y, sh.normal.z, 0);
+                    q.pathAov[3 * (uint64_t)slot + 1] = make_float4(sh.tangent.x, sh.tangent.y, sh.tangent.z, 0);
+                    q.pathAov[3 * (uint64_t)slot + 2] =
+                        make_float4(sh.bitangent.x, sh.bitangent.y, sh.bitangent.z, 0);
+                }
+                verts++;
+                rays += 1 + (vo.hasP ? 1 : 0);
+                // A shadow ray whose contribution is exactly zero cannot change L.
+                wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
+                wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
+                cont = !last;
+            }
+        }
+        int qi = waveAlloc(countOut, cont);
+        if (cont) {
+            float4* Q = rayOut + 2 * (uint64_t)qi;
+            Q[0] = make_float4(vo.nextO.x, vo.nextO.y, vo.nextO.z, __uint_as_float(slot));
+            Q[1] = make_float4(vo.nextD.x, vo.nextD.y, vo.nextD.z, 0.f);
+        }
+        int hi = waveAlloc(q.counters + 2, wantH);
+        if (wantH) {
+            float4* S = q.shadowH + 3 * (uint64_t)hi;
+            S[0] = make_float4(vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, __uint_as_float(slot));
+            S[1] = make_float4(vo.hDir.x, vo.hDir.y, vo.hDir.z, __int_as_float(0x7f800000));
+            S[2] = make_float4(vo.contribH.x, vo.contribH.y, vo.contribH.z, 0);
+        }
+        int pi = waveAlloc(q.counters + 3, wantP);
+        if (wantP) {
+            float4* S = q.shadowP + 3 * (uint64_t)pi;
+            S[0] = make_float4(vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, __uint_as_float(slot));
+            S[1] = make_float4(vo.pDir.x, vo.pDir.y, vo.pDir.z, vo.pTmax);
+            S[2] = make_float4(vo.contribP.x, vo.contribP.y, vo.contribP.z, 0);
+        }
+    }
+    flushCount(&counters[0], rays);
+    flushCount(&counters[1], verts);
+}
+
+template <int W>
+__global__ void k_wf_shadow(DevScene s, WfState q, const float4* __restrict__ sh, const uint32_t* countIn,
+                            unsigned long long* counters) {
+    TravStack st = makeStack(s, q.gstack, q.gthreads);
+    TraceCounters cnt{0, 0};
+    uint32_t n = *countIn;
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t overflow = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        float4 A = sh[3 * (uint64_t)i], B = sh[3 * (uint64_t)i + 1];
+        TraceResult h = traceRay<W, true, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, B.w, st, cnt);
+        overflow |= st.overflow;
+        if (h.tri < 0) {
+            float4 C = sh[3 * (uint64_t)i + 2];
+            uint32_t slot = __float_as_uint(A.w);
+            float4 L = q.pathL[slot];
+            q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
+        }
+    }
+    flushCount(&counters[3], overflow);
+}
+
+// One thread per distinct pixel of the generation; slots j, j+npad, ... hold
+// the same pixel, so accumulation needs no atomics.
+__global__ void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __restrict__ accum,
+                              uint32_t* __restrict__ counts, float4* __restrict__ aov,
+                              unsigned long long* counters) {
+    uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t lim = a.genCount < a.npad ? a.genCount : a.npad;
+    uint32_t paths = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < lim; j += stride) {
+        for (uint32_t slot = j; slot < a.genCount; slot += a.npad) {
+            float4 Lf = q.pathL[slot];
+            if (Lf.w == 0.f) continue;   // padding pixel
+            paths++;
+            uint32_t idx = q.pathPix[slot];
+            v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
+            if (anyNan(L)) continue;
+            float4 acc = accum[idx];
+            accum[idx] = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
+            counts[idx] += 1;
+            if (aov) {
+                float4* A = aov + 3 * (uint64_t)idx;
+                const float4* P = q.pathAov + 3 * (uint64_t)slot;
+                for (int k = 0; k < 3; k++) A[k] = make_float4(A[k].x + P[k].x, A[k].y + P[k].y, A[k].z + P[k].z, 0);
+            }
+        }
+    }
+    flushCount(&counters[2], paths);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+}  // namespace dev
+
+// ------------------------------------------------------------ host launch --
+using namespace dev;
+
+static uint32_t gridFor(uint64_t n, uint32_t block, uint32_t maxBlocks) {
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (uint32_t)(g < maxBlocks ? g : maxBlocks);
+}
+
+hipError_t launchTrace(const LaunchCfg& c, const DevScene& s, const float* rays, const float* tmax, uint32_t n,
+                       int32_t* tri, float* t, float* uv, uint8_t* occ, bool any, bool count,
+                       unsigned long long* counters) {
+    uint32_t grid = gridFor(n, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+#define EL_TRACE(WW, A, C)                                                                                   \
+    hipLaunchKernelGGL((k_trace<WW, A, C>), dim3(grid), dim3(c.block), lds, c.stream, s, rays, tmax, n, tri, t, \
+                       uv, occ, c.gstack, c.maxThreads, counters)
+    if (s.width == 4) {
+        if (any)
+            EL_TRACE(4, true, false);
+        else if (count)
+            EL_TRACE(4, false, true);
+        else
+            EL_TRACE(4, false, false);
+    } else {
+        if (any)
+            EL_TRACE(8, true, false);
+        else if (count)
+            EL_TRACE(8, false, true);
+        else
+            EL_TRACE(8, false, false);
+    }
+#undef EL_TRACE
+    return hipGetLastError();
+}
+
+hipError_t launchPlain(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, float4* accum, uint32_t* counts,
+                       float4* aov, unsigned long long* counters) {
+    uint64_t total = (uint64_t)((s.cam.xRes + 15) / 16) * ((s.cam.yRes + 15) / 16) * 256;
+    uint32_t grid = gridFor(total, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+    if (s.width == 4)
+        hipLaunchKernelGGL(k_plain<4>, dim3(grid), dim3(c.block), lds, c.stream, s, a, accum, counts, aov, c.gstack,
+                           c.maxThreads, counters);
+    else
+        hipLaunchKernelGGL(k_plain<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, accum, counts, aov, c.gstack,
+                           c.maxThreads, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchWfCamera(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, const WfState& q) {
+    uint32_t grid = gridFor(a.genCount, c.block, c.maxThreads / c.block);
+    hipLaunchKernelGGL(k_wf_camera, dim3(grid), dim3(c.block), 0, c.stream, s, a, q);
+    return hipGetLastError();
+}
+
+hipError_t launchWfExtend(const LaunchCfg& c, const DevScene& s, const WfState& q, const float4* rayIn,
+                          const uint32_t* countIn, int resetIdx, uint32_t maxCount, bool countTraversal,
+                          unsigned long long* counters) {
+    uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+    if (s.width == 4)
+        hipLaunchKernelGGL(k_wf_extend<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetIdx,
+                           countTraversal ? 1 : 0, counters);
+    else
+        hipLaunchKernelGGL(k_wf_extend<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetIdx,
+                           countTraversal ? 1 : 0, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchWfShade(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, const WfState& q,
+                         const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
+                         uint32_t maxCount, unsigned long long* counters) {
+    uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
+    hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(c.block), 0, c.stream, s, a, q, rayIn, countIn, rayOut,
+                       countOut, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchWfShadow(const LaunchCfg& c, const DevScene& s, const WfState& q, const float4* shadow,
+                          const uint32_t* countIn, uint32_t maxCount, unsigned long long* counters) {
+    uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+    if (s.width == 4)
+        hipLaunchKernelGGL(k_wf_shadow<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, shadow, countIn,
+                           counters);
+    else
+        hipLaunchKernelGGL(k_wf_shadow<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, shadow, countIn,
+                           counters);
+    return hipGetLastError();
+}
+
+hipError_t launchWfFinalize(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, const WfState& q,
+                            float4* accum, uint32_t* counts, float4* aov, unsigned long long* counters) {
+    uint32_t lim = a.genCount < a.npad ? a.genCount : a.npad;
+    uint32_t grid = gridFor(lim, c.block, c.maxThreads / c.block);
+    hipLaunchKernelGGL(k_wf_finalize, dim3(grid), dim3(c.block), 0, c.stream, s, a, q, accum, counts, aov,
+                       counters);
+    return hipGetLastError();
+}
+
+hipError_t launchFillU32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n) {
+    uint32_t grid = (uint32_t)((n + 255) / 256);
+    if (grid > 4096) grid = 4096;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace eleven

@@ -1,0 +1,63 @@
+// Host-visible kernel arguments and launchers (implemented in kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eleven {
+namespace dev {
+struct DevScene;
+}
+
+struct RenderArgs {
+    uint32_t k0, k1;        // RNG key (seed)
+    uint32_t sample;        // first sample index of this launch / generation
+    uint32_t scene;         // scene id (counter word 3)
+    uint32_t maxDepth;
+    uint32_t bounce;        // wavefront: current bounce
+    uint64_t genStart;      // wavefront: first work item of the generation
+    uint32_t genCount;      // wavefront: work items in the generation
+    uint32_t npad;          // wavefront: padded pixel count (8x8 tiles)
+};
+
+struct WfState {
+    float4* ray[2];         // ray queues: (o, slot) (d, -) per entry
+    float4* hits;           // (tri, t, u, v) per entry of the current queue
+    float4* shadowH;        // (o, slot) (d, tmax) (contribution) per entry
+    float4* shadowP;
+    float4* pathT;          // throughput per path slot
+    float4* pathL;          // radiance per path slot, w = live flag
+    uint32_t* pathPix;      // output pixel index per path slot
+    uint32_t* pathSample;   // sample index per path slot
+    float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
+    uint32_t* counters;     // [0],[1] ray queues, [2] HDRI shadows, [3] point shadows
+    int* gstack;
+    uint32_t gthreads;
+};
+
+struct LaunchCfg {
+    hipStream_t stream;
+    uint32_t block;
+    uint32_t maxThreads;    // grid * block upper bound (size of the spill area)
+    int* gstack;
+};
+
+hipError_t launchTrace(const LaunchCfg& c, const dev::DevScene& s, const float* rays, const float* tmax, uint32_t n,
+                       int32_t* tri, float* t, float* uv, uint8_t* occ, bool any, bool count,
+                       unsigned long long* counters);
+hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, float4* accum,
+                       uint32_t* counts, float4* aov, unsigned long long* counters);
+hipError_t launchWfCamera(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q);
+hipError_t launchWfExtend(const LaunchCfg& c, const dev::DevScene& s, const WfState& q, const float4* rayIn,
+                          const uint32_t* countIn, int resetIdx, uint32_t maxCount, bool countTraversal,
+                          unsigned long long* counters);
+hipError_t launchWfShade(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q,
+                         const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
+                         uint32_t maxCount, unsigned long long* counters);
+hipError_t launchWfShadow(const LaunchCfg& c, const dev::DevScene& s, const WfState& q, const float4* shadow,
+                          const uint32_t* countIn, uint32_t maxCount, unsigned long long* counters);
+hipError_t launchWfFinalize(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q,
+                            float4* accum, uint32_t* counts, float4* aov, unsigned long long* counters);
+hipError_t launchFillU32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
+
+}  // namespace eleven

@@ -1,0 +1,651 @@
+// Device-side core of the MI355X path tracer (gfx950, wave64).
+//
+// Estimator semantics follow the reference kernel (src/tfg-pathtracer/):
+//   camera ray          kernel.cu:260-337      hit attributes   Tri.hpp:38-160
+//   material fetch      kernel.cu:54-119       Disney           Disney.hpp:41-253
+//   sampling            Sampling.hpp:10-54     point light NEE  kernel.cu:175-205
+//   HDRI NEE            kernel.cu:210-258, HDRI.hpp:130-162, Texture.hpp:95-207
+//   MIS combine         kernel.cu:339-358      path loop        kernel.cu:369-481
+// Defined behaviour where the reference is undefined or replaced:
+//   * RNG: Philox4x32-10 (Salmon et al., SC'11) keyed by (seed, scene) with
+//     counter (pixel, sample, block, 0); uniforms in [0,1) from the top 24 bits.
+//   * The HDRI NEE pdf is computed whether or not the shadow ray is blocked
+//     (the reference leaves it uninitialised when blocked, kernel.cu:248/254).
+//   * Point-light visibility: blocked iff some triangle is hit at
+//     t < dist - 0.001 from the offset origin (the reference compares the
+//     closest hit's smooth-shading position, kernel.cu:193-197; the two differ
+//     only for an occluder within the shading offset of the light).
+//   * The HDRI pixel is drawn from an alias table of the reference search's
+//     effective distribution (bin from the 32-bit shade-r1 draw, acceptance
+//     from an extra draw); the point light uses the same r1 as the reference.
+//   * acos arguments are clamped to [-1, 1] for the environment lookup.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace eleven {
+namespace dev {
+
+#define EL_PI 3.14159265358979323846f
+
+// ---------------------------------------------------------------- vectors --
+struct v3 {
+    float x, y, z;
+};
+__device__ __forceinline__ v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+__device__ __forceinline__ v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ v3 operator*(v3 a, v3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ v3 operator*(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 operator*(float s, v3 a) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ v3 operator/(v3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ v3 operator/(v3 a, v3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+// Vector.hpp:229-231 form (y component negated difference).
+__device__ __forceinline__ v3 cross(v3 a, v3 b) {
+    return mk(a.y * b.z - a.z * b.y, -(a.x * b.z - a.z * b.x), a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float length(v3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Vector.hpp:173-176: zero vectors pass through unchanged.
+__device__ __forceinline__ v3 normalized(v3 a) {
+    float l = length(a);
+    if (l == 0.f) return a;
+    return a / l;
+}
+__device__ __forceinline__ float lerpf(float a, float b, float c) { return a + c * (b - a); }  // FAST_LERP
+__device__ __forceinline__ v3 lerp3(v3 a, v3 b, float c) {
+    return mk(lerpf(a.x, b.x, c), lerpf(a.y, b.y, c), lerpf(a.z, b.z, c));
+}
+__device__ __forceinline__ float clampf(float a, float b, float c) { return a < b ? b : (a > c ? c : a); }
+__device__ __forceinline__ float minf(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float maxf(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ v3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+
+// ------------------------------------------------------------------- RNG --
+// Philox4x32-10, Salmon, Moraes, Dror, Shaw: "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11.  Multipliers and Weyl key increments as published.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+__device__ __forceinline__ float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+// Dimension layout of one path: dims 0..4 camera (r1..r5), then per bounce
+// b the five dims 5+5b .. 9+5b: DisneySample r1,r2,r3; shade r1; alias accept.
+struct PathRng {
+    uint32_t k0, k1, pixel, sample, scene;
+    __device__ __forceinline__ uint4 block(uint32_t b) const {
+        return philox4x32_10(make_uint4(pixel, sample, b, scene), k0, k1);
+    }
+    // Draw dims [d, d+n) (n <= 5) into out[].
+    __device__ __forceinline__ void dims(uint32_t d, int n, uint32_t* out) const {
+        uint32_t b = d >> 2;
+        uint4 q = block(b);
+        uint32_t w[8] = {q.x, q.y, q.z, q.w, 0, 0, 0, 0};
+        if (((d & 3) + n) > 4) {
+            uint4 q2 = block(b + 1);
+            w[4] = q2.x;
+            w[5] = q2.y;
+            w[6] = q2.z;
+            w[7] = q2.w;
+        }
+        for (int i = 0; i < n; i++) out[i] = w[(d & 3) + i];
+    }
+};
+
+// ----------------------------------------------------------- scene layout --
+struct DevTex {
+    int32_t w, h;
+    float xTile, yTile, xOffset, yOffset;
+    int32_t filter;
+    int32_t pad;
+    uint64_t offset;   // first float of this texture in the texel pool
+};
+
+struct DevMaterial {
+    int32_t albedoTex, emissionTex, roughnessTex, metallicTex, normalTex, pad0;
+    float albedo[3], emission[3];
+    float roughness, metallic, clearcoatGloss, clearcoat, anisotropic, eta, transmission, specular,
+        specularTint, sheenTint, subsurface, sheen;
+};
+
+struct DevCamera {
+    uint32_t xRes, yRes;
+    float focalLength, sensorWidth, sensorHeight, aperture, focusDistance;
+    float rot[3], pos[3];
+    int32_t bokeh;
+};
+
+// Shading stream, 36 floats per original triangle.
+constexpr int kShadeStride = 36;
+// [0..8] v0 v1 v2 | [9..17] n0 n1 n2 | [18..23] uv0 uv1 uv2 (x,y) |
+// [24..32] t0 t1 t2 | [33] tangentsSign | [34] material id (int bits)
+
+struct DevScene {
+    const void* nodes;             // WideNode<W>[]
+    const float4* tris;            // 3 float4 per leaf-order triangle
+    const float* shade;            // kShadeStride floats per original triangle
+    const DevMaterial* mats;
+    const DevTex* texs;
+    const float* texels;           // pool of all textures (RGB FP32)
+    const float* lights;           // 6 floats per light: position, radiance
+    const float* envProb;          // alias acceptance per bin
+    const int32_t* envAlias;       // alias target per bin
+    DevTex env;                    // env texture (texels in the pool)
+    float envRadianceSum;
+    int32_t envLen;
+    int32_t nLights;
+    int32_t nTris;
+    int32_t width;                 // BVH width (4 or 8)
+    int32_t gcap;                  // global stack entries per thread
+    DevCamera cam;
+};
+
+// --------------------------------------------------------------- textures --
+// Texture::getValueFromCoordinates (Texture.hpp:95-106).
+__device__ __forceinline__ v3 texel(const float* __restrict__ pool, const DevTex& t, int x, int y) {
+    x = (int)(t.xTile * (x + t.xOffset * t.w)) % t.w;
+    y = (int)(t.yTile * (y + t.yOffset * t.h)) % t.h;
+    // Out-of-range coordinates (negative uv) are undefined in the reference;
+    // wrap them into the texture instead of reading out of bounds.
+    if (x < 0) x += t.w;
+    if (y < 0) y += t.h;
+    const float* p = pool + t.offset + 3 * ((uint64_t)y * t.w + x);
+    return ld3(p);
+}
+__device__ __forceinline__ v3 texUV(const float* pool, const DevTex& t, float u, float v) {
+    return texel(pool, t, (int)(u * t.w), (int)(v * t.h));
+}
+// Texture.hpp:114-135
+__device__ __forceinline__ v3 texBilinear(const float* pool, const DevTex& t, float u, float v) {
+    float x = u * t.w, y = v * t.h;
+    float t1x = floorf(x), t1y = floorf(y);
+    float t2x = t1x + 1, t2y = t1y + 1;
+    float a = (x - t1x) / (t2x - t1x), b = (y - t1y) / (t2y - t1y);
+    v3 v1 = texel(pool, t, (int)t1x, (int)t1y), v2 = texel(pool, t, (int)t2x, (int)t1y);
+    v3 v3_ = texel(pool, t, (int)t1x, (int)t2y), v4 = texel(pool, t, (int)t2x, (int)t2y);
+    return lerp3(lerp3(v1, v2, a), lerp3(v3_, v4, a), b);
+}
+__device__ __forceinline__ v3 texFiltered(const float* pool, const DevTex& t, float u, float v) {
+    return t.filter == 1 ? texBilinear(pool, t, u, v) : texUV(pool, t, u, v);
+}
+
+// Texture::sphericalMapping (Texture.hpp:144-156) for a unit sphere at origin.
+__device__ __forceinline__ void sphericalMapping(v3 p, float& u, float& v) {
+    float theta = acosf(clampf(-p.y, -1.f, 1.f));
+    float phi = atan2f(-p.z, p.x) + EL_PI;
+    u = phi / (2 * EL_PI);
+    v = theta / EL_PI;
+    // limitUV (Math.hpp:38-41)
+    u += -(float)(u > 1) + -(float)(u < 0);
+    v += -(float)(v > 1) + -(float)(v < 0);
+}
+
+// -------------------------------------------------------------- sampling --
+__device__ __forceinline__ v3 cosineSampleHemisphere(float u1, float u2) {   // Sampling.hpp:31-41
+    float r = sqrtf(u1);
+    float phi = 2.0f * EL_PI * u2;
+    v3 d;
+    d.x = r * cosf(phi);
+    d.y = r * sinf(phi);
+    d.z = sqrtf(maxf(0.0f, 1.0f - d.x * d.x - d.y * d.y));
+    return d;
+}
+__device__ __forceinline__ v3 importanceSampleGGX(float rgh, float r1, float r2) {   // Sampling.hpp:43-54
+    float a = maxf(0.001f, rgh);
+    float phi = r1 * EL_PI * 2;
+    float cosTheta = sqrtf((1.0f - r2) / (1.0f + (a * a - 1.0f) * r2));
+    float sinTheta = clampf(sqrtf(1.0f - (cosTheta * cosTheta)), 0.0f, 1.0f);
+    return mk(sinTheta * cosf(phi), sinTheta * sinf(phi), cosTheta);
+}
+__device__ __forceinline__ void uniformCircleSampling(float u1, float u2, float u3, float& x, float& y) {
+    float t = 2 * EL_PI * u1;
+    float u = u2 + u3;
+    float r = u > 1 ? 2 - u : u;
+    x = r * cosf(t);
+    y = r * sinf(t);
+}
+
+// ---------------------------------------------------------------- Disney --
+struct HitData {
+    float metallic, roughness, clearcoatGloss, clearcoat, anisotropic, eta, transmission, specular,
+        specularTint, sheenTint, subsurface, sheen;
+    v3 emission, albedo, normal, tangent, bitangent;
+};
+
+__device__ __forceinline__ void createBasis(v3 n, v3& t, v3& b) {   // Disney.hpp:41-45
+    t = normalized(cross(mk(0, 1, 0), n));
+    b = normalized(cross(n, t));
+}
+__device__ __forceinline__ float schlickFresnel(float u) {
+    float m = clampf(1.0f - u, 0.0f, 1.0f);
+    float m2 = m * m;
+    return m2 * m2 * m;
+}
+__device__ __forceinline__ float GTR1(float NDotH, float a) {
+    if (a >= 1.0f) return 1.0f / EL_PI;
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * NDotH * NDotH;
+    return (a2 - 1.0f) / (EL_PI * logf(a2) * t);
+}
+__device__ __forceinline__ float GTR2_aniso(float NDotH, float HDotX, float HDotY, float ax, float ay) {
+    float a = HDotX / ax, b = HDotY / ay;
+    float c = a * a + b * b + NDotH * NDotH;
+    return 1.0f / (EL_PI * ax * ay * c * c);
+}
+__device__ __forceinline__ float smithG_GGX(float NDotV, float alphaG) {
+    float a = alphaG * alphaG, b = NDotV * NDotV;
+    return 1.0f / (NDotV + sqrtf(a + b - a * b));
+}
+__device__ __forceinline__ float smithG_GGX_aniso(float NDotV, float VDotX, float VDotY, float ax, float ay) {
+    float a = VDotX * ax, b = VDotY * ay, c = NDotV;
+    return 1.0f / (NDotV + sqrtf(a * a + b * b + c * c));
+}
+
+// Disney.hpp:108-147.  `dir` is the incoming ray direction (V = -dir).
+__device__ __forceinline__ float disneyPdf(v3 dir, const HitData& h, v3 L) {
+    v3 N = h.normal;
+    v3 V = dir * -1.0f;
+    v3 H = normalized(L + V);
+    v3 T, B;
+    createBasis(h.normal, T, B);
+    float NDotH = fabsf(dot(N, H));
+    if (dot(N, L) <= 0.0f) return 1.0f;
+    float clearcoatAlpha = lerpf(0.1f, 0.001f, h.clearcoatGloss);
+    float diffuseRatio = 0.5f * (1.0f - h.metallic);
+    float specularRatio = 1.0f - diffuseRatio;
+    float aspect = sqrtf(1.0f - h.anisotropic * 0.9f);
+    float ax = maxf(0.001f, h.roughness / aspect);
+    float ay = maxf(0.001f, h.roughness * aspect);
+    float pdfGTR2 = GTR2_aniso(NDotH, dot(H, T), dot(H, B), ax, ay) * NDotH;
+    float pdfGTR1 = GTR1(NDotH, clearcoatAlpha) * NDotH;
+    float ratio = 1.0f / (1.0f + h.clearcoat);
+    float pdfSpec = lerpf(pdfGTR1, pdfGTR2, ratio) / (4.0f * fabsf(dot(L, H)));
+    float pdfDiff = fabsf(dot(L, N)) * (1.0f / EL_PI);
+    return diffuseRatio * pdfDiff + specularRatio * pdfSpec;
+}
+
+// Disney.hpp:150-177
+__device__ __forceinline__ v3 disneySample(v3 dir, const HitData& h, float r1, float r2, float r3) {
+    v3 N = h.normal;
+    v3 V = dir * -1.0f;
+    v3 T, B;
+    createBasis(h.normal, T, B);
+    float diffuseRatio = 0.5f * (1.0f - h.metallic);
+    if (r3 < diffuseRatio) {
+        v3 H = cosineSampleHemisphere(r1, r2);
+        return T * H.x + B * H.y + N * H.z;
+    }
+    v3 H = importanceSampleGGX(h.roughness, r1, r2);
+    H = T * H.x + B * H.y + N * H.z;
+    v3 v1 = V * -1.0f;
+    return v1 - 2 * dot(v1, H) * H;   // reflect (Vector.hpp:218-220)
+}
+
+// Disney.hpp:179-253
+__device__ __forceinline__ v3 disneyEval(v3 dir, const HitData& h, v3 L) {
+    v3 V = dir * -1.0f;
+    v3 T, B;
+    createBasis(h.normal, T, B);
+    v3 H = normalized(L + V);
+    float NDotL = fabsf(dot(h.normal, L));
+    float NDotV = fabsf(dot(h.normal, V));
+    float NDotH = fabsf(dot(h.normal, H));
+    float LDotH = fabsf(dot(L, H));
+    v3 brdf = mk(0, 0, 0);
+    if (h.transmission < 1.0f && dot(h.normal, L) > 0.0f && dot(h.normal, V) > 0.0f) {
+        v3 Cdlin = h.albedo;
+        float Cdlum = 0.3f * Cdlin.x + 0.6f * Cdlin.y + 0.1f * Cdlin.z;
+        v3 Ctint = Cdlum > 0.0f ? Cdlin / Cdlum : mk(1, 1, 1);
+        v3 Cspec0 = lerp3(h.specular * 0.08f * lerp3(mk(1, 1, 1), Ctint, h.specularTint), Cdlin, h.metallic);
+        v3 Csheen = lerp3(mk(1, 1, 1), Ctint, h.sheenTint);
+        float FL = schlickFresnel(NDotL), FV = schlickFresnel(NDotV);
+        float Fd90 = 0.5f + 2.0f * LDotH * LDotH * h.roughness;
+        float Fd = lerpf(1.0f, Fd90, FL) * lerpf(1.0f, Fd90, FV);
+        float Fss90 = LDotH * LDotH * h.roughness;
+        float Fss = lerpf(1.0f, Fss90, FL) * lerpf(1.0f, Fss90, FV);
+        float ss = 1.25f * (Fss * (1.0f / (NDotL + NDotV) - 0.5f) + 0.5f);
+        float aspect = sqrtf(1.0f - h.anisotropic * 0.9f);
+        float ax = maxf(0.001f, h.roughness / aspect);
+        float ay = maxf(0.001f, h.roughness * aspect);
+        float Ds = GTR2_aniso(NDotH, dot(H, T), dot(H, B), ax, ay);
+        float FH = schlickFresnel(LDotH);
+        v3 Fs = lerp3(Cspec0, mk(1, 1, 1), FH);
+        float Gs = smithG_GGX_aniso(NDotL, dot(L, T), dot(L, B), ax, ay);
+        Gs *= smithG_GGX_aniso(NDotV, dot(V, T), dot(V, B), ax, ay);
+        v3 Fsheen = FH * h.sheen * Csheen;
+        float Dr = GTR1(NDotH, lerpf(0.1f, 0.001f, h.clearcoatGloss));
+        float Fr = lerpf(0.04f, 1.0f, FH);
+        float Gr = smithG_GGX(NDotL, 0.25f) * smithG_GGX(NDotV, 0.25f);
+        brdf = ((1.0f / EL_PI) * lerpf(Fd, ss, h.subsurface) * Cdlin + Fsheen) * (1.0f - h.metallic) +
+               Gs * Fs * Ds + mk(1, 1, 1) * (0.25f * h.clearcoat * Gr * Fr * Dr);
+    }
+    return brdf;
+}
+
+// ----------------------------------------------------------------- camera --
+// kernel.cu:260-337 (double-promoted terms kept as in the reference).
+__device__ __forceinline__ void cameraRay(const DevCamera& c, int x, int y, float r1, float r2, float r3, float r4,
+                                          float r5, v3& o, v3& d) {
+    float dx = c.pos[0] + ((float)x) / ((float)c.xRes) * c.sensorWidth;
+    float dy = c.pos[1] + ((float)y) / ((float)c.yRes) * c.sensorHeight;
+    float odx = (float)((-c.sensorWidth / 2.0) + dx);
+    float ody = (float)((-c.sensorHeight / 2.0) + dy);
+    float rx = (float)((1.0 / (float)c.xRes) * (r1 - 0.5) * c.sensorWidth);
+    float ry = (float)((1.0 / (float)c.yRes) * (r2 - 0.5) * c.sensorHeight);
+    float SPx = odx + rx, SPy = ody + ry;
+    float SPz = c.pos[2] + c.focalLength;
+    v3 rot = mk(c.rot[0], c.rot[1], c.rot[2]);
+    rot = rot * (float)(EL_PI / 180.0);
+    v3 dir = mk(SPx, SPy, SPz) - mk(c.pos[0], c.pos[1], c.pos[2]);
+    v3 dX = mk(dir.x, dir.y * cosf(rot.x) - dir.z * sinf(rot.x), dir.y * sinf(rot.x) + dir.z * cosf(rot.x));
+    v3 dY = mk(dX.x * cosf(rot.y) + dX.z * sinf(rot.y), dX.y, dX.z * cosf(rot.y) - dX.x * sinf(rot.y));
+    v3 dZ = mk(dY.x * cosf(rot.z) - dY.y * sinf(rot.z), dY.x * sinf(rot.z) + dY.y * cosf(rot.z), dY.z);
+    o = mk(c.pos[0], c.pos[1], c.pos[2]);
+    d = normalized(dZ);
+    if (c.bokeh) {
+        float diameter = c.focalLength / c.aperture;
+        float l = c.focusDistance + c.focalLength;
+        v3 focus = o + d * l;
+        float px, py;
+        uniformCircleSampling(r3, r4, r5, px, py);
+        px *= diameter * 0.5f;
+        py *= diameter * 0.5f;
+        v3 orig = o + mk(px, py, 0);
+        o = orig;
+        d = normalized(focus - orig);
+    }
+}
+
+// --------------------------------------------------------------- hit data --
+struct SurfaceHit {
+    v3 position, normal, tangent, bitangent;
+    float tu, tv;
+    int32_t material;
+};
+
+// Tri::hit's attribute part (Tri.hpp:70-157) from the traversal's (t, u, v).
+__device__ __forceinline__ SurfaceHit surfaceAt(const DevScene& s, int tri, float t, float u, float v, v3 o, v3 d) {
+    const float* S = s.shade + (uint64_t)tri * kShadeStride;
+    SurfaceHit h;
+    v3 v0 = ld3(S + 0), v1 = ld3(S + 3), v2 = ld3(S + 6);
+    v3 n0 = ld3(S + 9), n1 = ld3(S + 12), n2 = ld3(S + 15);
+    float uv0x = S[18], uv0y = S[19], uv1x = S[20], uv1y = S[21], uv2x = S[22], uv2y = S[23];
+    v3 t0 = ld3(S + 24), t1 = ld3(S + 27), t2 = ld3(S + 30);
+    float sign = S[33];
+    h.material = __float_as_int(S[34]);
+    h.tu = uv0x + (uv1x - uv0x) * u + (uv2x - uv0x) * v;
+    h.tv = uv0y + (uv1y - uv0y) * u + (uv2y - uv0y) * v;
+    v3 geom = o + d * t;
+    v3 Ns = n0 + (n1 - n0) * u + (n2 - n0) * v;
+    v3 Ts = t0 + (t1 - t0) * u + (t2 - t0) * v;
+    v3 p0 = geom - dot(geom - v0, n0) * n0;
+    v3 p1 = geom - dot(geom - v1, n1) * n1;
+    v3 p2 = geom - dot(geom - v2, n2) * n2;
+    v3 sp = p0 + (p1 - p0) * u + (p2 - p0) * v;
+    bool convex = dot(sp - geom, Ns) > 0.0f;
+    h.position = convex ? sp : geom;
+    h.normal = Ns;
+    h.tangent = Ts;
+    h.bitangent = sign * cross(Ns, Ts);
+    return h;
+}
+
+// generateHitData (kernel.cu:54-119)
+__device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& h) {
+    const DevMaterial& m = s.mats[h.material];
+    HitData d;
+    d.albedo = m.albedoTex < 0 ? ld3(m.albedo) : texFiltered(s.texels, s.texs[m.albedoTex], h.tu, h.tv);
+    d.emission = m.emissionTex < 0 ? ld3(m.emission) : texFiltered(s.texels, s.texs[m.emissionTex], h.tu, h.tv);
+    d.roughness = m.roughnessTex < 0 ? m.roughness : texFiltered(s.texels, s.texs[m.roughnessTex], h.tu, h.tv).x;
+    d.metallic = m.metallicTex < 0 ? m.metallic : texFiltered(s.texels, s.texs[m.metallicTex], h.tu, h.tv).x;
+    if (m.normalTex < 0) {
+        d.normal = h.normal;
+    } else {
+        v3 c = texUV(s.texels, s.texs[m.normalTex], h.tu, h.tv);
+        v3 ln = c * 2.0f - mk(1, 1, 1);
+        d.normal = normalized(ln.x * h.tangent - ln.y * h.bitangent + ln.z * h.normal);
+    }
+    d.roughness = powf(d.roughness, 2.2f);
+    d.metallic = powf(d.metallic, 2.2f);
+    d.clearcoatGloss = m.clearcoatGloss;
+    d.clearcoat = m.clearcoat;
+    d.anisotropic = m.anisotropic;
+    d.eta = m.eta;
+    d.transmission = m.transmission;
+    d.specular = m.specular;
+    d.specularTint = m.specularTint;
+    d.sheenTint = m.sheenTint;
+    d.subsurface = m.subsurface;
+    d.sheen = m.sheen;
+    d.tangent = h.tangent;
+    d.bitangent = h.bitangent;
+    return d;
+}
+
+// Escaped ray: env(-dir), nearest/filtered lookup (kernel.cu:414-419).
+__device__ __forceinline__ v3 envLookup(const DevScene& s, v3 d) {
+    float u, v;
+    sphericalMapping(d * -1.0f, u, v);
+    return texFiltered(s.texels, s.env, u, v);
+}
+
+// HDRI NEE sample (kernel.cu:236-254 with HDRI.hpp:145-162, Texture.hpp:177-207).
+struct EnvSample {
+    v3 dir, value;
+    float pdf;
+};
+__device__ __forceinline__ EnvSample envSample(const DevScene& s, uint32_t ubin, float accept) {
+    uint32_t bin = (uint32_t)(((uint64_t)ubin * (uint64_t)s.envLen) >> 32);
+    int k = accept < s.envProb[bin] ? (int)bin : s.envAlias[bin];
+    const DevTex& t = s.env;
+    int x = k % t.w, y = k / t.w;
+    float nu = x / (float)t.w, nv = y / (float)t.h;
+    // inverseTransformUV
+    int ix = (int)(nu * t.w), iy = (int)(nv * t.h);
+    ix = (int)(t.xTile * (ix - t.xOffset * t.w)) % t.w;
+    iy = (int)(t.yTile * (iy - t.yOffset * t.h)) % t.h;
+    float iu = (float)ix / (float)t.w, iv = (float)iy / (float)t.h;
+    iu += -(float)(iu > 1) + -(float)(iu < 0);
+    iv += -(float)(iv > 1) + -(float)(iv < 0);
+    // reverseSphericalMapping, negated and normalised
+    float phi = iu * 2 * EL_PI, theta = iv * EL_PI;
+    float px = cosf(phi - EL_PI), py = -cosf(theta), pz = -sinf(phi - EL_PI);
+    float a = sqrtf(1 - py * py);
+    EnvSample e;
+    e.dir = normalized(mk(a * px, py, a * pz)) * -1.0f;
+    e.value = texUV(s.texels, t, iu, iv);
+    // HDRI::pdf((int)(iu*W), (int)(iv*H))
+    int qx = (int)(iu * t.w), qy = (int)(iv * t.h);
+    v3 dv = texel(s.texels, t, qx, qy);
+    float th = (((float)qy / (float)t.h)) * EL_PI;
+    e.pdf = ((dv.x + dv.y + dv.z) / s.envRadianceSum) * (float)(t.w * t.h) / (2.0f * EL_PI * sinf(th));
+    return e;
+}
+
+// ------------------------------------------------------------- traversal --
+// Moller-Trumbore exactly as Tri::hit (Tri.hpp:38-68): FP32, no contraction,
+// IEEE division for 1/det, inclusive barycentric bounds, t >= 0.
+__device__ __forceinline__ bool mtHit(float4 A, float4 B, float4 C, float ox, float oy, float oz, float dx, float dy,
+                                      float dz, float& t, float& u, float& v) {
+#pragma clang fp contract(off)
+    const float EPS = 0.0000001f;
+    float e1x = A.w, e1y = B.x, e1z = B.y;
+    float e2x = B.z, e2y = B.w, e2z = C.x;
+    float px = dy * e2z - dz * e2y;
+    float py = -(dx * e2z - dz * e2x);
+    float pz = dx * e2y - dy * e2x;
+    float det = e1x * px + e1y * py + e1z * pz;
+    if (det > -EPS && det < EPS) return false;
+    float inv = 1.0f / det;
+    float tx = ox - A.x, ty = oy - A.y, tz = oz - A.z;
+    u = (tx * px + ty * py + tz * pz) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    float qx = ty * e1z - tz * e1y;
+    float qy = -(tx * e1z - tz * e1x);
+    float qz = tx * e1y - ty * e1x;
+    v = (dx * qx + dy * qy + dz * qz) * inv;
+    if (v < 0.0f || (u + v) > 1.0f) return false;
+    t = (e2x * qx + e2y * qy + e2z * qz) * inv;
+    if (t < 0.0f) return false;
+    return true;
+}
+
+constexpr int kShortStack = 16;   // LDS entries per lane (power of two)
+
+// Per-lane traversal stack: the top kShortStack entries live in LDS
+// (lane-interleaved, lds[k*blockDim + tid]); older entries spill to a global
+// per-thread area (gstack[k*gstride]).  The host sizes the global part from
+// the tree's worst-case depth so it never overflows; if it would, the entry is
+// dropped and `overflow` is set (no out-of-bounds access).
+struct TravStack {
+    int* lds;
+    int ldsStride;
+    int* g;
+    int gstride;
+    int gcap;
+    int sp, lo;
+    bool overflow;
+    __device__ __forceinline__ void push(int e) {
+        if (sp - lo == kShortStack) {
+            int old = lds[(lo & (kShortStack - 1)) * ldsStride];
+            if (lo < gcap)
+                g[(uint64_t)lo * gstride] = old;
+            else
+                overflow = true;
+            lo++;
+        }
+        lds[(sp & (kShortStack - 1)) * ldsStride] = e;
+        sp++;
+    }
+    __device__ __forceinline__ int pop() {
+        if (sp == 0) return -1;
+        sp--;
+        if (sp >= lo) return lds[(sp & (kShortStack - 1)) * ldsStride];
+        lo = sp;
+        return sp < gcap ? g[(uint64_t)sp * gstride] : -1;
+    }
+};
+
+// Same layout as eleven::WideNode<W> (bvh.hpp).
+template <int W>
+struct alignas(16) DNode {
+    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
+    int32_t child[W];
+    int32_t pad[W];
+};
+
+struct TraceResult {
+    int32_t tri;   // original index, -1 if none
+    float t, u, v;
+};
+
+struct TraceCounters {
+    uint32_t nodes, tris;
+};
+
+template <int W, bool ANY, bool COUNT>
+__device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, float oy, float oz, float dx, float dy,
+                                                float dz, float tmax, TravStack& st, TraceCounters& cnt) {
+    using Node = DNode<W>;
+    const Node* __restrict__ nodes = (const Node*)s.nodes;
+    const float4* __restrict__ tris = s.tris;
+    const float tiny = 1e-30f;
+    float ix = 1.0f / (fabsf(dx) > tiny ? dx : copysignf(tiny, dx));
+    float iy = 1.0f / (fabsf(dy) > tiny ? dy : copysignf(tiny, dy));
+    float iz = 1.0f / (fabsf(dz) > tiny ? dz : copysignf(tiny, dz));
+    const bool nx = ix < 0, ny = iy < 0, nz = iz < 0;
+    TraceResult r;
+    r.tri = -1;
+    r.t = tmax;
+    r.u = r.v = 0.f;
+    st.sp = 0;
+    st.lo = 0;
+    int cur = 0;
+    while (cur != -1) {
+        if (cur >= 0) {
+            const Node& n = nodes[cur];
+            if (COUNT) cnt.nodes++;
+            float tcull = ANY ? r.t : r.t * (1.0f + 1.0f / 1048576.0f);
+            uint32_t keys[W];
+#pragma unroll
+            for (int i = 0; i < W; i++) {
+                float bx0 = nx ? n.hix[i] : n.lox[i], bx1 = nx ? n.lox[i] : n.hix[i];
+                float by0 = ny ? n.hiy[i] : n.loy[i], by1 = ny ? n.loy[i] : n.hiy[i];
+                float bz0 = nz ? n.hiz[i] : n.loz[i], bz1 = nz ? n.loz[i] : n.hiz[i];
+                float tn = fmaxf(fmaxf((bx0 - ox) * ix, (by0 - oy) * iy), fmaxf((bz0 - oz) * iz, 0.0f));
+                float tf = fminf(fminf((bx1 - ox) * ix, (by1 - oy) * iy), fminf((bz1 - oz) * iz, tcull));
+                keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+            }
+            // sort ascending (small networks)
+            if (W == 4) {
+#define EL_CSWAP(a, b)                         \
+    {                                          \
+        uint32_t lo_ = min(keys[a], keys[b]);  \
+        uint32_t hi_ = max(keys[a], keys[b]);  \
+        keys[a] = lo_;                         \
+        keys[b] = hi_;                         \
+    }
+                EL_CSWAP(0, 1) EL_CSWAP(2, 3) EL_CSWAP(0, 2) EL_CSWAP(1, 3) EL_CSWAP(1, 2)
+            } else {
+#pragma unroll
+                for (int i = 1; i < W; i++)
+#pragma unroll
+                    for (int j = i; j > 0; j--) EL_CSWAP(j - 1, j)
+#undef EL_CSWAP
+            }
+            int nh = 0;
+#pragma unroll
+            for (int i = 0; i < W; i++) nh += keys[i] != 0xFFFFFFFFu;
+            if (nh == 0) {
+                cur = st.pop();
+            } else {
+#pragma unroll
+                for (int i = W - 1; i >= 1; i--)
+                    if (i < nh) st.push(n.child[keys[i] & 7u]);
+                cur = n.child[keys[0] & 7u];
+            }
+        } else {
+            int k = -cur - 2;
+            int first = k >> 4, count = (k & 15) + 1;
+            bool done = false;
+            for (int i = 0; i < count; i++) {
+                const float4* p = tris + (uint64_t)(first + i) * 3;
+                float4 A = p[0], B = p[1], C = p[2];
+                if (COUNT) cnt.tris++;
+                float t, u, v;
+                if (mtHit(A, B, C, ox, oy, oz, dx, dy, dz, t, u, v)) {
+                    int id = __float_as_int(C.y);
+                    if (ANY) {
+                        if (t < r.t) {
+                            r.tri = id;
+                            r.t = t;
+                            r.u = u;
+                            r.v = v;
+                            done = true;
+                            break;
+                        }
+                    } else if (t < r.t || (t == r.t && (unsigned)id < (unsigned)r.tri)) {
+                        r.tri = id;
+                        r.t = t;
+                        r.u = u;
+                        r.v = v;
+                    }
+                }
+            }
+            cur = done ? -1 : st.pop();
+        }
+    }
+    return r;
+}
+
+}  // namespace dev
+}  // namespace eleven
