@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Throughput of the path-tracing hot path on one MI355X.
+
+Workload (one "step"): the 1,048,576-triangle heightfield scene at
+1920x1080, 64 samples per pixel, depth 8, defocus aperture 0.05, 16 textured
+materials, 8 point lights and a 2048x1024 HDR sky, rendered from scratch
+(accumulation cleared) by the wavefront renderer.  Inputs (BVH, triangles,
+textures, HDRI tables) are resident in HBM before the timed region.
+
+metric: Mrays/s counting every BVH query of the estimator (camera +
+extension + HDRI shadow + point-light shadow), the same logical count for
+the plain and wavefront renderers.
+
+Extra fields: `plain` (the one-thread-per-pixel megakernel on the same
+frame), `roofline` of the dominant kernel (closest-hit extension), and
+`cpu_baseline` (the CPU oracle, a restatement of the reference estimator,
+on a bounded crop of the same frame using all host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "tfg-pathtracer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+S_NODE = {4: 128, 8: 256}   # bytes per wide node as stored
+S_TRI = 48                  # bytes per triangle record as stored
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ensure_built():
+    import eleven
+    if not eleven.LIB_PATH.exists():
+        import __graft_entry__
+        __graft_entry__.build()
+
+
+def cpu_baseline(scene, budget_s=15.0, threads=None):
+    """Oracle render (FP32, reference estimator) of a centred crop of the frame,
+    its sample count chosen from a short probe so the timed part takes about
+    `budget_s` seconds on `threads` host cores."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_py as orc
+    threads = threads or min(16, os.cpu_count() or 1)
+    cam = scene.camera
+    t0 = time.perf_counter()
+    ref = orc.RefBVH(scene.vertices())
+    build_s = time.perf_counter() - t0
+    w, h = max(1, cam.x_res // 8), max(1, cam.y_res // 8)
+    x0, y0 = cam.x_res // 2 - w // 2, cam.y_res // 2 - h // 2
+    win = (x0, x0 + w, y0, y0 + h)
+    t0 = time.perf_counter()
+    orc.render(scene, ref, 1, depth=8, seed=0, window=win, threads=threads)
+    probe = max(time.perf_counter() - t0, 1e-3)
+    spp = max(1, min(4096, int(budget_s / probe)))
+    t0 = time.perf_counter()
+    r = orc.render(scene, ref, spp, depth=8, seed=0, s0=1, window=win, threads=threads)
+    dt = time.perf_counter() - t0
+    rays = r[3]
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{w}x{h} centre crop of the same frame, {spp} spp, depth 8, {rays} rays in {dt:.1f} s "
+                      f"(FP32 restatement of the reference estimator over the reference-faithful depth-18 / "
+                      f"14-bin SAH binary BVH; tree build {build_s:.1f} s not timed)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", choices=["wavefront", "plain"], default="wavefront")
+    ap.add_argument("--spp", type=int, default=64)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--bvh", type=int, default=0, help="BVH width 4 or 8 (0: library default)")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--queue", type=int, default=0)
+    ap.add_argument("--res", default="1920x1080")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--save", default="")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is "
+              f"a design note, not a measured configuration.", file=sys.stderr)
+        sys.exit(2)
+
+    ensure_built()
+    import eleven
+    from eleven import scenegen
+
+    res = tuple(int(x) for x in args.res.split("x"))
+    t0 = time.perf_counter()
+    d = scenegen.cached("heightfield", res=res)
+    log(f"scene {d} ready in {time.perf_counter() - t0:.1f} s")
+    scene = eleven.SceneData.load(d)
+    ctx = eleven.Context(0)
+    t0 = time.perf_counter()
+    ctx.upload(scene)
+    st0 = ctx.stats()
+    log(f"upload (BVH build + copy) {time.perf_counter() - t0:.2f} s, {scene.tri_count} tris")
+    mode = eleven.MODE_WAVEFRONT if args.mode == "wavefront" else eleven.MODE_PLAIN
+    kw = dict(depth=args.depth, mode=mode, bvh_width=args.bvh, block=args.block, queue_cap=args.queue)
+
+    for w in range(args.warmup):
+        ctx.clear()
+        st = ctx.render(args.spp, seed=1000 + w, **kw)
+        log(f"warmup {w}: {st.render_ms:.1f} ms, {st.rays / st.render_ms / 1e3:.1f} Mrays/s")
+
+    rays = 0
+    ext_ms = 0.0
+    ext_launches = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.clear()
+        st = ctx.render(args.spp, seed=k, **kw)
+        rays += st.rays
+        ext_ms += st.extend_ms
+        ext_launches += st.extend_launches
+        log(f"step {k}: {st.render_ms:.1f} ms, {st.rays} rays")
+    wall = time.perf_counter() - t0
+    value = rays / wall / 1e6
+    st = ctx.stats()
+    width = st.bvh_width
+    if args.save:
+        from eleven import imageio
+        img = ctx.read()[0]
+        imageio.write_png(args.save, imageio.to_ldr(img))
+
+    out = {"metric": "Mrays/s all bounces (wavefront vs plain, equal relMSE); adjoint grads/s",
+           "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 2), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"heightfield {scene.tri_count} tris, {res[0]}x{res[1]}, {args.spp} spp, "
+                                  f"depth {args.depth}, defocus f/0.05, 16 textured materials, 8 point lights, "
+                                  f"2048x1024 HDR sky",
+                      "renderer": args.mode, "bvh_width": width, "rays_per_step": rays // max(1, args.steps)}}
+
+    # roofline of the dominant kernel: closest-hit extension (wavefront only)
+    if mode == eleven.MODE_WAVEFRONT and not args.no_count and ext_launches:
+        ctx.clear()
+        sc = ctx.render(args.spp, seed=0, flags=eleven.FLAG_COUNT_TRAVERSAL, **kw)
+        per_frame_launches = sc.extend_launches
+        algo_bytes = sc.node_fetches * S_NODE[width] + sc.tri_tests * S_TRI
+        bytes_per_launch = algo_bytes / max(1, per_frame_launches)
+        avg_ms = ext_ms / ext_launches
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        ext_rays = sc.rays - 0  # informational
+        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "kernel": "k_wf_extend (closest-hit traversal)",
+                           "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_frame_launches,
+                           "algo_bytes_per_launch": int(bytes_per_launch),
+                           "node_fetches_per_step": sc.node_fetches, "tri_tests_per_step": sc.tri_tests,
+                           "bytes_per_node": S_NODE[width], "bytes_per_tri": S_TRI}
+        out["config"]["extend_ms_per_step"] = round(ext_ms / args.steps, 2)
+        out["config"]["shade_ms_per_step"] = round(st.shade_ms, 2)
+        out["config"]["shadow_ms_per_step"] = round(st.shadow_ms, 2)
+        del ext_rays
+
+    if not args.no_plain and mode == eleven.MODE_WAVEFRONT:
+        ctx.clear()
+        t1 = time.perf_counter()
+        sp = ctx.render(args.spp, seed=0, **{**kw, "mode": eleven.MODE_PLAIN})
+        pw = time.perf_counter() - t1
+        out["plain"] = {"value": round(sp.rays / pw / 1e6, 2), "ms_per_step": round(pw * 1e3, 2),
+                        "rays_per_step": sp.rays, "wavefront_speedup": round(value / (sp.rays / pw / 1e6), 3)}
+
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(scene)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -112,7 +112,7 @@ typedef struct {
     uint32_t max_depth;    /* MAXBOUNCES (Definitions.h:9); 0 -> 5          */
     uint64_t seed;         /* RNG key                                       */
     int32_t  mode;         /* EL_MODE_*                                     */
-    uint32_t block;        /* threads per block, 0 -> tuned default         */
+    uint32_t block;        /* threads per block: 64..256 step 64, 0 -> 256   */
     uint32_t queue_cap;    /* wavefront paths in flight, 0 -> default       */
     uint32_t bvh_width;    /* 4 or 8, 0 -> current                          */
     uint32_t flags;        /* EL_FLAG_*                                     */
@@ -148,6 +148,11 @@ int  el_scene_view_get(const el_scene* scene, el_scene_view* out);
  * (HDRI.hpp:107-128): cdf[w*h+1] and the FP32 radiance sum. */
 int  el_scene_hdri_cdf(const el_scene* scene, const float** cdf, float* radiance_sum);
 void el_scene_free(el_scene* scene);
+/* The renderer's environment-pixel sampler evaluated on the host: for each
+ * r[i] in [0, 1), pixel[i] = HDRI::sample's index binarySearch(cdf, r, W*H)
+ * (HDRI.hpp:130-162), computed with the same guide-table code the kernels
+ * run.  For inspection and tests; el_upload_scene builds the same tables. */
+int  el_hdri_lookup(const el_texture* hdri, const float* r, uint32_t n, int32_t* pixel);
 const char* el_scene_last_error(void);
 
 /* ---- device context ---------------------------------------------------- */

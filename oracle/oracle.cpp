@@ -335,16 +335,14 @@ inline void philox(const uint32_t c_[4], const uint32_t k_[2], uint32_t out[4]) 
     std::memcpy(out, c, sizeof c);
 }
 inline float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
-// dims [d, d+n) of the path (pixel, sample): block b = counter (pixel, sample, b, 0)
-inline void pathDims(uint32_t k0, uint32_t k1, uint32_t pixel, uint32_t sample, uint32_t d, int n, uint32_t* out) {
+// Random block b of path (pixel, sample): Philox counter (pixel, sample, b, 0).
+// Block 0: camera r1..r4; block 1 word 0: camera r5; block 2+i: bounce i
+// (DisneySample r1, r2, r3, shade r1).  Shade's r2/r3 are never read on the
+// reference's HDRI-IS path (kernel.cu:234-257).
+inline void pathBlock(uint32_t k0, uint32_t k1, uint32_t pixel, uint32_t sample, uint32_t b, uint32_t* out) {
     uint32_t key[2] = {k0, k1};
-    for (int i = 0; i < n; i++) {
-        uint32_t dd = d + i;
-        uint32_t c[4] = {pixel, sample, dd >> 2, 0};
-        uint32_t w[4];
-        philox(c, key, w);
-        out[i] = w[dd & 3];
-    }
+    uint32_t c[4] = {pixel, sample, b, 0};
+    philox(c, key, out);
 }
 
 // ------------------------------------------------------------ Disney -----
@@ -670,8 +668,10 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
     uint32_t W = cam.x_res, Hh = cam.y_res;
     uint32_t idx = W * (Hh - y - 1) + x;
     uint32_t k0 = (uint32_t)p.seed, k1 = (uint32_t)(p.seed >> 32);
-    uint32_t c[5];
-    pathDims(k0, k1, idx, sample, 0, 5, c);
+    uint32_t c[5], c1[4];
+    pathBlock(k0, k1, idx, sample, 0, c);
+    pathBlock(k0, k1, idx, sample, 1, c1);
+    c[4] = c1[0];
 
     // calculateCameraRay (kernel.cu:260-337)
     V3<R> o, d;
@@ -721,8 +721,8 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
         }
         Surf<R> sh = surface<R>(s, hit.tri, hit.t, hit.u, hit.v, o, d);
         HitData<R> hd = hitData<R>(s, sh);
-        uint32_t r[5];
-        pathDims(k0, k1, idx, sample, 5 + 5 * i, 5, r);
+        uint32_t r[4];
+        pathBlock(k0, k1, idx, sample, 2 + i, r);
         V3<R> bdir = DisneySample<R>(d, hd, u01(r[0]), u01(r[1]), u01(r[2]));
 
         // shade (kernel.cu:339-358)
@@ -734,14 +734,7 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
         V3<R> Hl;
         {
             const Tex& e = s.env;
-            int k;
-            if (p.hdri_mode == 0) {
-                k = binarySearch(s.cdf.data(), (float)r1, e.w * e.h);
-            } else {
-                uint32_t L_ = (uint32_t)(e.w * e.h);
-                uint32_t bin = (uint32_t)(((uint64_t)r[3] * L_) >> 32);
-                k = u01(r[4]) < p.alias_prob[bin] ? (int)bin : p.alias_idx[bin];
-            }
+            int k = binarySearch(s.cdf.data(), (float)r1, e.w * e.h);   // HDRI::sample
             int tx = k % e.w, ty = k / e.w;
             R nu = tx / (R)e.w, nv = ty / (R)e.h;
             int ix = (int)(nu * e.w), iy = (int)(nv * e.h);   // inverseTransformUV (Texture.hpp:177-193)
@@ -1066,7 +1059,6 @@ void or_camera_ray(const el_camera* cam, int x, int y, const float* r5, float* o
 int or_render(const el_scene_view* v, void* refbvh, const or_render_params* p, double* sum, double* sumsq,
               uint32_t* count, uint64_t* rays, uint64_t* vertices) {
     if (!v || !refbvh || !p || !sum || !sumsq || !count) return -1;
-    if (p->hdri_mode == 1 && (!p->alias_prob || !p->alias_idx)) return -1;
     if (p->x0 < 0 || p->y0 < 0 || p->x1 > (int)v->camera.x_res || p->y1 > (int)v->camera.y_res || p->x0 >= p->x1 ||
         p->y0 >= p->y1)
         return -1;

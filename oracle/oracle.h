@@ -70,11 +70,9 @@ float or_hdri_pdf(const float* rgb, int w, int h, float x_offset, float radiance
 /* calculateCameraRay (kernel.cu:260-337). */
 void or_camera_ray(const el_camera* cam, int x, int y, const float* r5, float* o3, float* d3);
 
-/* Full estimator (kernel.cu:369-481).
+/* Full estimator (kernel.cu:369-481), HDRI pixel by the reference's
+ * binarySearch over the CDF (HDRI.hpp:130-162).
  *   precision   0: FP32 as the reference, 1: FP64 shading and accumulation
- *   hdri_mode   0: reference binarySearch over the CDF, 1: alias table
- *               (alias_prob/alias_idx, bin from the 32-bit shade-r1 draw and
- *               acceptance from the extra draw, as the GPU)
  *   shadow_mode 0: reference point-light visibility (closest hit, smooth
  *               shading position closer than the light); 1: any hit with
  *               t < dist - 0.001 (the build's definition)
@@ -85,10 +83,8 @@ void or_camera_ray(const el_camera* cam, int x, int y, const float* r5, float* o
 typedef struct {
     uint32_t spp, s0, depth;
     uint64_t seed;
-    int precision, hdri_mode, shadow_mode, threads;
+    int precision, shadow_mode, threads;
     int x0, x1, y0, y1;
-    const float* alias_prob;
-    const int32_t* alias_idx;
 } or_render_params;
 
 int or_render(const el_scene_view* scene, void* refbvh, const or_render_params* p, double* sum, double* sumsq,

@@ -19,9 +19,8 @@ _lib = None
 
 class or_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("s0", C.c_uint32), ("depth", C.c_uint32), ("seed", C.c_uint64),
-                ("precision", C.c_int), ("hdri_mode", C.c_int), ("shadow_mode", C.c_int), ("threads", C.c_int),
-                ("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int),
-                ("alias_prob", C.POINTER(C.c_float)), ("alias_idx", C.POINTER(C.c_int32))]
+                ("precision", C.c_int), ("shadow_mode", C.c_int), ("threads", C.c_int),
+                ("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int)]
 
 
 def lib():
@@ -67,9 +66,10 @@ def _f32(a):
 
 
 def tri_hit(v9, o, d):
+    # keep every temporary array referenced for the duration of the call
+    a, b, c = _f32(v9), _f32(o), _f32(d)
     t, u, v = C.c_float(), C.c_float(), C.c_float()
-    h = lib().or_tri_hit(_f32(v9).ctypes.data, _f32(o).ctypes.data, _f32(d).ctypes.data, C.byref(t), C.byref(u),
-                         C.byref(v))
+    h = lib().or_tri_hit(a.ctypes.data, b.ctypes.data, c.ctypes.data, C.byref(t), C.byref(u), C.byref(v))
     return bool(h), t.value, u.value, v.value
 
 
@@ -136,25 +136,26 @@ def hitdata(normal=(0, 0, 1), albedo=(0.5, 0.5, 0.5), emission=(0, 0, 0), **kw):
     return np.array([d[k] for k in HD_FIELDS] + list(emission) + list(albedo) + list(normal), np.float64)
 
 
+def _f64(a):
+    return np.ascontiguousarray(a, np.float64)
+
+
 def disney_eval(hd, direction, L, precision=1):
     out = np.zeros(3)
-    lib().or_disney_eval(np.ascontiguousarray(hd, np.float64).ctypes.data,
-                         np.ascontiguousarray(direction, np.float64).ctypes.data,
-                         np.ascontiguousarray(L, np.float64).ctypes.data, precision, out.ctypes.data)
+    h, d, l = _f64(hd), _f64(direction), _f64(L)
+    lib().or_disney_eval(h.ctypes.data, d.ctypes.data, l.ctypes.data, precision, out.ctypes.data)
     return out
 
 
 def disney_pdf(hd, direction, L, precision=1):
-    return lib().or_disney_pdf(np.ascontiguousarray(hd, np.float64).ctypes.data,
-                               np.ascontiguousarray(direction, np.float64).ctypes.data,
-                               np.ascontiguousarray(L, np.float64).ctypes.data, precision)
+    h, d, l = _f64(hd), _f64(direction), _f64(L)
+    return lib().or_disney_pdf(h.ctypes.data, d.ctypes.data, l.ctypes.data, precision)
 
 
 def disney_sample(hd, direction, r1, r2, r3, precision=1):
     out = np.zeros(3)
-    lib().or_disney_sample(np.ascontiguousarray(hd, np.float64).ctypes.data,
-                           np.ascontiguousarray(direction, np.float64).ctypes.data, r1, r2, r3, precision,
-                           out.ctypes.data)
+    h, d = _f64(hd), _f64(direction)
+    lib().or_disney_sample(h.ctypes.data, d.ctypes.data, r1, r2, r3, precision, out.ctypes.data)
     return out
 
 
@@ -186,7 +187,8 @@ def hdri_cdf(rgb):
 
 
 def hdri_search(cdf, r, length):
-    return lib().or_hdri_search(_f32(cdf).ctypes.data, float(r), length)
+    c = _f32(cdf)
+    return lib().or_hdri_search(c.ctypes.data, float(r), length)
 
 
 def hdri_pdf(rgb, x_offset, radiance_sum, x, y):
@@ -198,12 +200,12 @@ def hdri_pdf(rgb, x_offset, radiance_sum, x, y):
 def camera_ray(cam, x, y, r5):
     o = np.zeros(3, np.float32)
     d = np.zeros(3, np.float32)
-    lib().or_camera_ray(C.byref(cam), x, y, _f32(r5).ctypes.data, o.ctypes.data, d.ctypes.data)
+    r = _f32(r5)
+    lib().or_camera_ray(C.byref(cam), x, y, r.ctypes.data, o.ctypes.data, d.ctypes.data)
     return o, d
 
 
-def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, hdri_mode=0, shadow_mode=1, threads=0,
-           window=None, alias=None):
+def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1, threads=0, window=None):
     """Oracle render of pixels `window` = (x0, x1, y0, y1) (kernel x, y).
 
     Returns (sum[H,W,3], sumsq[H,W,3], count[H,W], rays, vertices) in output
@@ -213,16 +215,8 @@ def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, hdri_mode=0, 
     x0, x1, y0, y1 = window or (0, W, 0, H)
     p = or_render_params()
     p.spp, p.s0, p.depth, p.seed = spp, s0, depth, seed
-    p.precision, p.hdri_mode, p.shadow_mode, p.threads = precision, hdri_mode, shadow_mode, threads
+    p.precision, p.shadow_mode, p.threads = precision, shadow_mode, threads
     p.x0, p.x1, p.y0, p.y1 = x0, x1, y0, y1
-    keep = []
-    if hdri_mode == 1:
-        prob, idx = alias
-        prob = _f32(prob)
-        idx = np.ascontiguousarray(idx, np.int32)
-        keep += [prob, idx]
-        p.alias_prob = prob.ctypes.data_as(C.POINTER(C.c_float))
-        p.alias_idx = idx.ctypes.data_as(C.POINTER(C.c_int32))
     s = np.zeros((H, W, 3))
     ss = np.zeros((H, W, 3))
     cnt = np.zeros((H, W), np.uint32)

@@ -1,0 +1,175 @@
+"""Image parity of the HIP renderers (plain megakernel and wavefront).
+
+1. plain == wavefront: same estimator and random numbers, so per-pixel
+   accepted-sample counts and the ray/vertex tallies are identical and the
+   images agree to FP32 summation order (rtol 1e-4).
+2. GPU vs oracle with common random numbers: the oracle replays the same
+   Philox streams, the same HDRI pixel choice and the same shadow rule, so
+   almost every pixel agrees to 1e-3 relative (paths may diverge where FP32
+   transcendental results differ in the last bit).
+3. GPU vs the FP64 reference image (tests/golden/ref_*.npz: the oracle with
+   the reference's own semantics -- binarySearch HDRI sampling, the
+   shading-position point-light test -- at 4096 spp).  The GPU renders 16
+   independent batches, whose spread gives the per-pixel variance of its
+   mean.  Passing means: measured relMSE = mean((I - I_ref)^2 / (I_ref^2 +
+   1e-2)) is consistent with the relMSE the two variances predict (ratio in
+   [0.75, 1.33]), no 8x8 block mean is off by more than 5 standard errors,
+   and at most 0.2 % of pixel channels exceed |z| = 5.
+"""
+import numpy as np
+import pytest
+
+import eleven
+import oracle_py as orc
+from render_scenes import DEPTH, SCENES, make_scene
+from pathlib import Path
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def scenes(scene_root):
+    return {k: make_scene(k, scene_root) for k in SCENES}
+
+
+@pytest.fixture(scope="module")
+def refbvhs(scenes):
+    return {k: orc.RefBVH(s.vertices()) for k, s in scenes.items()}
+
+
+def _render(ctx, scene, spp, seed, mode, depth=5, flags=0):
+    ctx.upload(scene)
+    st = ctx.render(spp, depth=depth, seed=seed, mode=mode, flags=flags)
+    img, cnt, rays, verts = ctx.read()
+    return img[..., :3].astype(np.float64), cnt, rays, verts, st
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", SCENES)
+def test_plain_equals_wavefront(gpu_ctx, scenes, kind):
+    print(f"[render] plain vs wavefront: {kind}", flush=True)
+    s = scenes[kind]
+    a, ca, ra, va, _ = _render(gpu_ctx, s, 16, 3, eleven.MODE_PLAIN)
+    b, cb, rb, vb, _ = _render(gpu_ctx, s, 16, 3, eleven.MODE_WAVEFRONT)
+    assert np.array_equal(ca, cb), "accepted-sample counts differ"
+    assert ra == rb and va == vb, f"tallies differ: rays {ra} vs {rb}, vertices {va} vs {vb}"
+    np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-6)
+    assert ca.min() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", SCENES)
+def test_gpu_matches_oracle_common_random_numbers(gpu_ctx, scenes, refbvhs, kind):
+    print(f"[render] GPU vs oracle, common random numbers: {kind}", flush=True)
+    s = scenes[kind]
+    spp, seed, depth = 8, 5, 5
+    g, gc, grays, gverts, _ = _render(gpu_ctx, s, spp, seed, eleven.MODE_WAVEFRONT, depth=depth)
+    osum, _, ocnt, orays, overts = orc.render(s, refbvhs[kind], spp, depth=depth, seed=seed, precision=0,
+                                                shadow_mode=1)
+    om = osum / np.maximum(ocnt, 1)[..., None]
+    assert abs(grays - orays) <= 0.005 * orays and abs(gverts - overts) <= 0.005 * overts
+    assert np.mean(gc == ocnt) > 0.99
+    rel = np.abs(g - om) / (np.abs(om) + 1e-3)
+    frac = np.mean(rel.max(-1) < 1e-3)
+    print(f"  pixels within 1e-3: {frac:.4f}, mean rel diff {rel.mean():.2e}", flush=True)
+    assert frac > 0.97
+    assert rel.mean() < 2e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", SCENES)
+def test_gpu_vs_fp64_reference_image(gpu_ctx, scenes, kind):
+    print(f"[render] GPU vs FP64 reference image (relMSE consistency): {kind}", flush=True)
+    ref = np.load(GOLDEN / f"ref_{kind}.npz")
+    I_ref, var_ref, n_ref = ref["mean"].astype(np.float64), ref["var"].astype(np.float64), float(ref["spp"])
+    s = scenes[kind]
+    gpu_ctx.upload(s)
+    batches, spp = 16, 64
+    means = []
+    for b in range(batches):
+        gpu_ctx.clear()
+        gpu_ctx.render(spp, depth=DEPTH, seed=1000 + b, mode=eleven.MODE_WAVEFRONT)
+        img, cnt, _, _ = gpu_ctx.read()
+        means.append(img[..., :3].astype(np.float64))
+    M = np.stack(means)
+    g = M.mean(0)
+    var_g_mean = M.var(0, ddof=1) / batches           # variance of the GPU mean
+    var_r_mean = var_ref / n_ref
+    w = 1.0 / (I_ref ** 2 + 1e-2)
+    measured = np.mean((g - I_ref) ** 2 * w)
+    predicted = np.mean((var_g_mean + var_r_mean) * w)
+    ratio = measured / predicted
+    se = np.sqrt(var_g_mean + var_r_mean)
+    ok = se > 1e-9
+    assert np.all(np.abs((g - I_ref)[~ok]) < 1e-5), "deterministic pixels differ"
+    z = (g - I_ref)[ok] / se[ok]
+    H, W = I_ref.shape[:2]
+    bz = []
+    for y in range(0, H - 7, 8):
+        for x in range(0, W - 7, 8):
+            d = (g - I_ref)[y:y + 8, x:x + 8].mean()
+            # channels of one pixel share a path: bound their covariance by full correlation
+            sd = np.sqrt(var_g_mean + var_r_mean)[y:y + 8, x:x + 8].sum(-1)
+            v = (sd ** 2).sum() / (64 * 3) ** 2
+            bz.append(d / np.sqrt(v) if v > 0 else 0.0)
+    bz = np.abs(np.array(bz))
+    print(f"  relMSE measured {measured:.3e} predicted {predicted:.3e} ratio {ratio:.3f}; "
+          f"|z|>5 {np.mean(np.abs(z) > 5):.4f}; max block |z| {bz.max():.2f}", flush=True)
+    assert 0.75 < ratio < 1.33
+    assert bz.max() < 5
+    assert np.mean(np.abs(z) > 5) < 0.002
+
+
+@pytest.mark.gpu
+def test_aov_normal_pass(gpu_ctx, scenes):
+    print("[render] AOV passes", flush=True)
+    s = scenes["spheres"]
+    gpu_ctx.upload(s)
+    gpu_ctx.render(4, seed=1, flags=eleven.FLAG_AOV)
+    n = gpu_ctx.read_aov(eleven.PASS_NORMAL)[..., :3]
+    # bottom rows of the frame see the ground plane first (vn 0 1 0)
+    bottom = n[-3:, :, :].reshape(-1, 3)
+    ground = np.abs(bottom[:, 1] - 1) < 1e-4
+    assert ground.mean() > 0.5
+    assert np.all(np.abs(bottom[ground, 0]) < 1e-5) and np.all(np.abs(bottom[ground, 2]) < 1e-5)
+    with pytest.raises(eleven.ElevenError):
+        gpu_ctx.read_aov(eleven.PASS_BEAUTY)
+
+
+@pytest.mark.gpu
+def test_error_paths(gpu_ctx, scenes):
+    print("[render] error paths", flush=True)
+    ctx = eleven.Context(0)
+    with pytest.raises(eleven.ElevenError) as e:
+        ctx.render(1)
+    assert e.value.code == eleven.EL_ERR_STATE
+    ctx.upload(scenes["spheres"])
+    with pytest.raises(eleven.ElevenError) as e:
+        ctx.render(1, bvh_width=6)
+    assert e.value.code == eleven.EL_ERR_ARG
+    with pytest.raises(eleven.ElevenError) as e:
+        ctx.render(1, block=100)
+    assert e.value.code == eleven.EL_ERR_ARG
+    with pytest.raises(eleven.ElevenError) as e:
+        ctx.render(1, depth=65)
+    assert e.value.code == eleven.EL_ERR_RANGE
+    st = ctx.render(0)               # spp = 0 is a no-op
+    img, cnt, rays, _ = ctx.read()
+    assert cnt.sum() == 0 and rays == 0
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_progressive_accumulation(gpu_ctx, scenes):
+    """Two renders of 4 spp equal one render of 8 spp (sample indices continue)."""
+    print("[render] progressive accumulation", flush=True)
+    s = scenes["spheres"]
+    gpu_ctx.upload(s)
+    gpu_ctx.render(4, seed=9)
+    gpu_ctx.render(4, seed=9)
+    a, ca, ra, _ = gpu_ctx.read()
+    gpu_ctx.clear()
+    gpu_ctx.render(8, seed=9)
+    b, cb, rb, _ = gpu_ctx.read()
+    assert np.array_equal(ca, cb) and ra == rb
+    np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)

@@ -54,7 +54,14 @@ struct DevBuf {
     }
 };
 
-constexpr int kCounterSlots = 8;   // [0] rays [1] vertices [2] paths [3] overflow [4] nodes [5] tris
+constexpr int kCounterWords = kXcds * kStatSlots;   // per-XCD copies of the tallies (kernels.hpp)
+
+// Sum of the per-XCD copies of tally `slot`.
+inline unsigned long long tally(const unsigned long long* h, int slot) {
+    unsigned long long s = 0;
+    for (int x = 0; x < kXcds; x++) s += h[x * kStatSlots + slot];
+    return s;
+}
 
 }  // namespace
 
@@ -79,10 +86,10 @@ struct el_ctx {
     std::vector<dev::DevMaterial> mats;
     std::vector<dev::DevTex> texs;
 
-    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvProb, dEnvAlias;
+    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap;
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
     DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wAov, wCounters;
-    size_t wfCap = 0;
+    size_t wfCap = 0, wfQcap = 0;
     bool wfAov = false;
 
     DevBuf tRays, tTmax, tTri, tT, tUV, tOcc;   // scratch for host-pointer ray queries
@@ -151,7 +158,7 @@ int el_create(el_ctx** out, int device) {
     // resident-thread capacity: 32 waves of 64 lanes per CU
     c->maxThreads = (uint32_t)c->cus * 2048u;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
-    if (c->dCounters.ensure(sizeof(unsigned long long) * kCounterSlots) != hipSuccess) return EL_ERR_HIP;
+    if (c->dCounters.ensure(sizeof(unsigned long long) * kCounterWords) != hipSuccess) return EL_ERR_HIP;
     *out = c.release();
     return EL_OK;
 }
@@ -317,14 +324,10 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     // HDRI CDF exactly as HDRI::generateCDF at construction (offsets still 0),
     // then the alias table of the search's effective distribution.
     const el_texture& h = v->hdri;
-    size_t L = (size_t)h.width * h.height;
-    std::vector<float> cdf(L + 1, 0.f);
-    float sum = 0;
-    for (size_t i = 0; i < L; i++) sum += h.data[3 * i] + h.data[3 * i + 1] + h.data[3 * i + 2];
-    for (size_t i = 0; i < L; i++) cdf[i + 1] = cdf[i] + (h.data[3 * i] + h.data[3 * i + 1] + h.data[3 * i + 2]) / sum;
-    HdriTable tab = buildHdriTable(cdf.data(), (int)L);
-    EL_HIP(c, upload(c->dEnvProb, tab.prob.data(), tab.prob.size() * sizeof(float), c->stream));
-    EL_HIP(c, upload(c->dEnvAlias, tab.alias.data(), tab.alias.size() * sizeof(int32_t), c->stream));
+    HdriSampler hs = buildHdriSampler(h.data, h.width, h.height);
+    EL_HIP(c, upload(c->dEnvCdf, hs.cdf.data(), hs.cdf.size() * sizeof(float), c->stream));
+    EL_HIP(c, upload(c->dEnvGuide, hs.guide.data(), hs.guide.size() * sizeof(int32_t), c->stream));
+    EL_HIP(c, upload(c->dEnvMap, hs.map2.data(), hs.map2.size() * sizeof(int32_t), c->stream));
 
     // accumulation
     size_t npx = (size_t)v->camera.x_res * v->camera.y_res;
@@ -338,11 +341,12 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     s.texs = c->dTexs.as<dev::DevTex>();
     s.texels = c->dTexels.as<float>();
     s.lights = c->dLights.as<float>();
-    s.envProb = c->dEnvProb.as<float>();
-    s.envAlias = c->dEnvAlias.as<int32_t>();
+    s.envCdf = c->dEnvCdf.as<float>();
+    s.envGuide = c->dEnvGuide.as<int32_t>();
+    s.envMap = c->dEnvMap.as<int32_t>();
     s.env = env;
-    s.envRadianceSum = sum;
-    s.envLen = (int32_t)L;
+    s.envRadianceSum = hs.radianceSum;
+    s.envLog2G = hs.log2G;
     s.nLights = (int32_t)v->light_count;
     s.nTris = (int32_t)v->tri_count;
     dev::DevCamera& dc = s.cam;
@@ -353,9 +357,17 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     dc.sensorHeight = v->camera.sensor_height;
     dc.aperture = v->camera.aperture;
     dc.focusDistance = v->camera.focus_distance;
-    dc.rot[0] = v->camera.rotation.x;
-    dc.rot[1] = v->camera.rotation.y;
-    dc.rot[2] = v->camera.rotation.z;
+    // rotation *= (PI / 180.0) as Vector3 *= float (kernel.cu:301), then the
+    // six trig values the reference evaluates per ray (kernel.cu:304-306)
+    const float toRad = (float)(3.14159265358979323846f / 180.0);
+    const float rx = v->camera.rotation.x * toRad, ry = v->camera.rotation.y * toRad,
+                rz = v->camera.rotation.z * toRad;
+    dc.cosRot[0] = std::cos(rx);
+    dc.sinRot[0] = std::sin(rx);
+    dc.cosRot[1] = std::cos(ry);
+    dc.sinRot[1] = std::sin(ry);
+    dc.cosRot[2] = std::cos(rz);
+    dc.sinRot[2] = std::sin(rz);
     dc.pos[0] = v->camera.position.x;
     dc.pos[1] = v->camera.position.y;
     dc.pos[2] = v->camera.position.z;
@@ -384,21 +396,24 @@ static hipEvent_t poolEvent(el_ctx* c, size_t i) {
     return c->evPool[i];
 }
 
-static int ensureWf(el_ctx* c, size_t cap, bool aov) {
-    if (cap <= c->wfCap && (!aov || c->wfAov)) return EL_OK;
-    EL_HIP(c, c->wRay0.ensure(cap * 32));
-    EL_HIP(c, c->wRay1.ensure(cap * 32));
-    EL_HIP(c, c->wHits.ensure(cap * 16));
-    EL_HIP(c, c->wShH.ensure(cap * 48));
-    EL_HIP(c, c->wShP.ensure(cap * 48));
+// Path slots: `cap`; queue entries: max(cap, kXcds * segSize) (the camera
+// queue is indexed by slot, the compacted queues by XCD segment).
+static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
+    if (cap <= c->wfCap && qcap <= c->wfQcap && (!aov || c->wfAov)) return EL_OK;
+    EL_HIP(c, c->wRay0.ensure(qcap * 32));
+    EL_HIP(c, c->wRay1.ensure(qcap * 32));
+    EL_HIP(c, c->wHits.ensure(qcap * 16));
+    EL_HIP(c, c->wShH.ensure(qcap * 48));
+    EL_HIP(c, c->wShP.ensure(qcap * 48));
     EL_HIP(c, c->wT.ensure(cap * 16));
     EL_HIP(c, c->wL.ensure(cap * 16));
     EL_HIP(c, c->wPix.ensure(cap * 4));
     EL_HIP(c, c->wSample.ensure(cap * 4));
     if (aov) EL_HIP(c, c->wAov.ensure(cap * 48));
-    EL_HIP(c, c->wCounters.ensure(64));
-    c->wfCap = cap;
-    c->wfAov = aov;
+    EL_HIP(c, c->wCounters.ensure(kQCounters * sizeof(uint32_t)));
+    c->wfCap = std::max(c->wfCap, cap);
+    c->wfQcap = std::max(c->wfQcap, qcap);
+    c->wfAov = c->wfAov || aov;
     return EL_OK;
 }
 
@@ -407,7 +422,8 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
     if (p->mode != EL_MODE_PLAIN && p->mode != EL_MODE_WAVEFRONT) return fail(c, EL_ERR_ARG, "unknown mode");
     if (p->bvh_width != 0 && p->bvh_width != 4 && p->bvh_width != 8) return fail(c, EL_ERR_ARG, "bvh_width must be 4 or 8");
-    if (p->block != 0 && (p->block % 64 != 0 || p->block > 1024)) return fail(c, EL_ERR_ARG, "block must be a multiple of 64, <= 1024");
+    if (p->block != 0 && (p->block % 64 != 0 || p->block > (uint32_t)kMaxBlock))
+        return fail(c, EL_ERR_ARG, "block must be a multiple of 64, at most 256");
     uint32_t depth = p->max_depth ? p->max_depth : 5;
     if (depth > 64) return fail(c, EL_ERR_RANGE, "max_depth above 64");
     EL_HIP(c, hipSetDevice(c->device));
@@ -426,7 +442,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     c->aovOn = c->aovOn || aov;
     float4* aovBuf = aov ? c->dAov.as<float4>() : nullptr;
     unsigned long long* ctr = c->dCounters.as<unsigned long long>();
-    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterSlots, c->stream));
+    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, c->stream));
     LaunchCfg lc = cfgOf(c);
     RenderArgs a{};
     a.k0 = (uint32_t)p->seed;
@@ -458,7 +474,11 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
             Q = std::max<uint64_t>(64, (Q / 64) * 64);
         if (Q > total) Q = total;
         if (Q > 0x7fffffffull) return fail(c, EL_ERR_RANGE, "queue capacity above 2^31");
-        int rc = ensureWf(c, Q, aov);
+        // Entries one XCD's blocks can append in one launch: at most
+        // ceil(Q/8) + maxThreads/8 + block (grid-stride over <= maxThreads).
+        const uint64_t segSize = (Q + kXcds - 1) / kXcds + lc.maxThreads / kXcds + lc.block;
+        const uint64_t qcap = std::max<uint64_t>(Q, kXcds * segSize);
+        int rc = ensureWf(c, Q, qcap, aov);
         if (rc) return rc;
         WfState q{};
         q.ray[0] = c->wRay0.as<float4>();
@@ -472,6 +492,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         q.pathSample = c->wSample.as<uint32_t>();
         q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
         q.counters = c->wCounters.as<uint32_t>();
+        q.segSize = (uint32_t)segSize;
         q.gstack = lc.gstack;
         q.gthreads = lc.maxThreads;
         a.npad = (uint32_t)npad;
@@ -480,29 +501,28 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         for (uint64_t start = 0; start < total; start += Q) {
             a.genStart = start;
             a.genCount = (uint32_t)std::min<uint64_t>(Q, total - start);
-            EL_HIP(c, launchFillU32(c->stream, q.counters, 0, 4));
             EL_HIP(c, launchWfCamera(lc, c->ds, a, q));
             for (uint32_t b = 0; b < depth; b++) {
                 a.bounce = b;
-                int cur = b & 1, nxt = cur ^ 1;
+                const int cur = b & 1, nxt = cur ^ 1;
+                uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
+                uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
                 hipEvent_t x0 = poolEvent(c, ev), x1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(x0, c->stream));
-                EL_HIP(c, launchWfExtend(lc, c->ds, q, q.ray[cur], q.counters + cur, nxt, a.genCount, countTrav,
-                                         ctr));
+                EL_HIP(c, launchWfExtend(lc, c->ds, q, q.ray[cur], cntCur, cntNxt, a.genCount, countTrav, ctr));
                 EL_HIP(c, hipEventRecord(x1, c->stream));
                 timed.push_back({ev, 0});
                 ev += 2;
                 hipEvent_t y0 = poolEvent(c, ev), y1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(y0, c->stream));
-                EL_HIP(c, launchWfShade(lc, c->ds, a, q, q.ray[cur], q.counters + cur, q.ray[nxt],
-                                        q.counters + nxt, a.genCount, ctr));
+                EL_HIP(c, launchWfShade(lc, c->ds, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, a.genCount, ctr));
                 EL_HIP(c, hipEventRecord(y1, c->stream));
                 timed.push_back({ev, 1});
                 ev += 2;
                 hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(z0, c->stream));
-                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowH, q.counters + 2, a.genCount, ctr));
-                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowP, q.counters + 3, a.genCount, ctr));
+                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowH, q.counters + kQShadowH, a.genCount, ctr));
+                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowP, q.counters + kQShadowP, a.genCount, ctr));
                 EL_HIP(c, hipEventRecord(z1, c->stream));
                 timed.push_back({ev, 2});
                 ev += 2;
@@ -525,25 +545,25 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     }
     float totalMs = 0;
     EL_HIP(c, hipEventElapsedTime(&totalMs, e0, e1));
-    unsigned long long h[kCounterSlots];
+    unsigned long long h[kCounterWords];
     EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
     EL_HIP(c, hipStreamSynchronize(c->stream));
     c->sampleBase += p->spp;
-    c->totalRays += h[0];
-    c->totalVerts += h[1];
-    c->totalPaths += h[2];
-    c->stats.rays = h[0];
-    c->stats.vertices = h[1];
-    c->stats.paths = h[2];
+    c->totalRays += tally(h, kStatRays);
+    c->totalVerts += tally(h, kStatVerts);
+    c->totalPaths += tally(h, kStatPaths);
+    c->stats.rays = tally(h, kStatRays);
+    c->stats.vertices = tally(h, kStatVerts);
+    c->stats.paths = tally(h, kStatPaths);
     c->stats.render_ms = totalMs;
     c->stats.extend_ms = extendMs;
     c->stats.shade_ms = shadeMs;
     c->stats.shadow_ms = shadowMs;
     c->stats.extend_launches = extendLaunches;
     c->stats.extend_rays = 0;
-    c->stats.node_fetches = h[4];
-    c->stats.tri_tests = h[5];
-    if (h[3]) return fail(c, EL_ERR_RANGE, "traversal stack overflow (results incomplete)");
+    c->stats.node_fetches = tally(h, kStatNodes);
+    c->stats.tri_tests = tally(h, kStatTris);
+    if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack or queue overflow (results incomplete)");
     return EL_OK;
 }
 
@@ -690,9 +710,9 @@ static int traceCommon(el_ctx* c, const el_ray* rays, const float* tmax, uint32_
         dT = c->tT.as<float>();
     }
     unsigned long long* ctr = c->dCounters.as<unsigned long long>();
-    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterSlots, c->stream));
+    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, c->stream));
     EL_HIP(c, launchTrace(cfgOf(c), c->ds, dR, dTm, n, dTri, dT, dUV, dOcc, any, count, ctr));
-    unsigned long long h[kCounterSlots];
+    unsigned long long h[kCounterWords];
     EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
     if (!devp) {
         if (any) {
@@ -704,9 +724,9 @@ static int traceCommon(el_ctx* c, const el_ray* rays, const float* tmax, uint32_
         }
     }
     EL_HIP(c, hipStreamSynchronize(c->stream));
-    if (nodes) *nodes = h[0];
-    if (tris) *tris = h[1];
-    if (h[2]) return fail(c, EL_ERR_RANGE, "traversal stack overflow (results incomplete)");
+    if (nodes) *nodes = tally(h, kStatNodes);
+    if (tris) *tris = tally(h, kStatTris);
+    if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack overflow (results incomplete)");
     return EL_OK;
 }
 

@@ -5,8 +5,8 @@
 namespace eleven {
 
 int hdriSearchIndex(int j, int length) {
-    // The reference loop (HDRI.hpp:130-142) with its comparisons resolved for
-    // a value strictly between cdf[j] and cdf[j+1].
+    // The reference loop with its comparisons resolved for a value strictly
+    // between cdf[j] and cdf[j+1]: `value < arr[m]` <=> m > j.
     int from = 0, to = length - 1;
     while (to - from > 0) {
         int m = from + (to - from) / 2;
@@ -18,52 +18,47 @@ int hdriSearchIndex(int j, int length) {
     return to;
 }
 
-static double clamp01(double x) { return x < 0 ? 0 : (x > 1 ? 1 : x); }
+int hdriBinarySearch(const float* arr, float value, int length) {
+    int from = 0, to = length - 1;
+    while (to - from > 0) {
+        int m = from + (to - from) / 2;
+        if (value == arr[m]) return m;
+        if (value < arr[m]) to = m - 1;
+        if (value > arr[m]) from = m + 1;
+    }
+    return to;
+}
 
-HdriTable buildHdriTable(const float* cdf, int L) {
-    HdriTable t;
-    t.pmf.assign((size_t)L, 0.0);
+HdriSampler buildHdriSampler(const float* rgb, int w, int h) {
+    HdriSampler s;
+    const int L = w * h;
+    s.length = L;
+    s.cdf.assign((size_t)L + 1, 0.f);
+    float sum = 0;
+    for (int i = 0; i < L; i++) sum += rgb[3 * (size_t)i] + rgb[3 * (size_t)i + 1] + rgb[3 * (size_t)i + 2];
+    for (int i = 0; i < L; i++)
+        s.cdf[i + 1] = s.cdf[i] + (rgb[3 * (size_t)i] + rgb[3 * (size_t)i + 1] + rgb[3 * (size_t)i + 2]) / sum;
+    s.radianceSum = sum;
+
+    s.map2.resize(2 * (size_t)L);
     for (int j = 0; j < L; j++) {
-        double a = clamp01((double)cdf[j]);
-        double b = (j == L - 1) ? 1.0 : clamp01((double)cdf[j + 1]);
-        double m = b > a ? b - a : 0.0;
-        if (m > 0) t.pmf[(size_t)hdriSearchIndex(j, L)] += m;
+        s.map2[2 * (size_t)j] = hdriSearchIndex(j, L);
+        s.map2[2 * (size_t)j + 1] = hdriBinarySearch(s.cdf.data(), s.cdf[j], L);
     }
-    double total = 0;
-    for (double p : t.pmf) total += p;
-    if (total <= 0) {   // degenerate (all-black) map: uniform, matching g over equal intervals
-        for (auto& p : t.pmf) p = 1.0 / L;
-        total = 1.0;
-    }
-    for (auto& p : t.pmf) p /= total;
 
-    // Vose's alias method.
-    t.prob.assign((size_t)L, 1.f);
-    t.alias.resize((size_t)L);
-    std::vector<double> scaled((size_t)L);
-    std::vector<int32_t> small, large;
-    small.reserve(L);
-    large.reserve(L);
-    for (int i = 0; i < L; i++) {
-        scaled[i] = t.pmf[i] * L;
-        t.alias[i] = i;
-        (scaled[i] < 1.0 ? small : large).push_back(i);
+    int k = 0;
+    while ((1 << k) < L) k++;
+    s.log2G = k;
+    const int G = 1 << k;
+    s.guide.resize((size_t)G + 1);
+    // guide[c] = max{ j < L : cdf[j] <= c / G }; c / G is exact in FP32.
+    int j = 0;
+    for (int c = 0; c <= G; c++) {
+        float edge = (float)c / (float)G;
+        while (j + 1 < L && s.cdf[j + 1] <= edge) j++;
+        s.guide[c] = j;
     }
-    while (!small.empty() && !large.empty()) {
-        int s = small.back();
-        small.pop_back();
-        int l = large.back();
-        t.prob[s] = (float)scaled[s];
-        t.alias[s] = l;
-        scaled[l] = (scaled[l] + scaled[s]) - 1.0;
-        if (scaled[l] < 1.0) {
-            large.pop_back();
-            small.push_back(l);
-        }
-    }
-    for (int i : large) t.prob[i] = 1.f;
-    for (int i : small) t.prob[i] = 1.f;
-    return t;
+    return s;
 }
 
 }  // namespace eleven

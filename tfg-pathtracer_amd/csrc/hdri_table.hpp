@@ -1,32 +1,68 @@
-// HDRI importance sampling as an alias table over the *effective* pixel
-// distribution of the reference's sampler.
+// HDRI importance sampling that reproduces the reference's sampler exactly.
 //
-// The reference draws a pixel with HDRI::sample(r1) (HDRI.hpp:154-162), i.e.
-// binarySearch(cdf, r1, W*H) (HDRI.hpp:130-142).  That search is not a lower
-// bound: for r strictly inside (cdf[j], cdf[j+1]) every comparison
-// `r < cdf[m]` is equivalent to `m > j` (the CDF is non-decreasing), so the
-// returned pixel is a function g(j) of the interval index alone.  The
-// effective probability of pixel k is therefore the sum of the interval
-// masses cdf[j+1]-cdf[j] over all j with g(j) = k, with the last interval
-// extending to 1.  The table reproduces that distribution exactly; drawing
-// the bin uses the full 32-bit uniform and the acceptance test a second one.
+// The reference draws an environment pixel with HDRI::sample(r1)
+// (HDRI.hpp:154-162) = binarySearch(cdf, r1, W*H) (HDRI.hpp:130-142), and
+// draws the point light from the same r1 (kernel.cu:347-348, 185).  Because
+// each path's radiance is clamped to [0, 10] before accumulation
+// (kernel.cu:447), the estimator's expectation depends on the joint
+// distribution of (pixel, light), so a sampler with the same pixel marginal
+// but a different map r1 -> pixel (an alias table) changes the image.  This
+// sampler returns binarySearch(cdf, r1, L) bit for bit, in O(1) expected
+// memory accesses:
+//
+//   * that search is not a lower bound.  For r strictly inside
+//     (cdf[j], cdf[j+1]) every comparison `r < cdf[m]` is `m > j` (the CDF is
+//     non-decreasing), so the result is a function open[j] of the interval
+//     index; for r == cdf[j] exactly the result is tie[j], the search run on
+//     that value.  Both tables are precomputed per interval.
+//   * the interval j(r) = max{ j < L : cdf[j] <= r } is found with a guide
+//     table over G = 2^k >= L equal cells of [0, 1) (the cutpoint method,
+//     Chen & Asau 1974): guide[c] = j(c / G), then a binary search inside
+//     [guide[c], guide[c+1]], which is usually empty or one step.
 #pragma once
 
 #include <cstdint>
 #include <vector>
 
+#ifdef __HIPCC__
+#define EL_HD __host__ __device__
+#else
+#define EL_HD
+#endif
+
 namespace eleven {
 
-// g(j): pixel returned by the reference search for any r in (cdf[j], cdf[j+1]).
+// open[j]: result of the reference search for any r in (cdf[j], cdf[j+1]).
 int hdriSearchIndex(int j, int length);
+// The reference loop itself (HDRI.hpp:130-142), used for the tie table.
+int hdriBinarySearch(const float* cdf, float value, int length);
 
-struct HdriTable {
-    std::vector<float> prob;    // acceptance probability per bin
-    std::vector<int32_t> alias; // alias pixel per bin
-    std::vector<double> pmf;    // effective pixel pmf (sums to 1)
+struct HdriSampler {
+    std::vector<float> cdf;       // L + 1 entries, HDRI::generateCDF layout
+    std::vector<int32_t> guide;   // G + 1 entries
+    std::vector<int32_t> map2;    // [2j] = open[j], [2j+1] = tie[j]
+    float radianceSum = 0;
+    int32_t length = 0;           // L = W * H
+    int32_t log2G = 0;
 };
 
-// cdf has length+1 entries (HDRI::generateCDF layout).
-HdriTable buildHdriTable(const float* cdf, int length);
+// From the RGB texels: the CDF and radiance sum exactly as HDRI::generateCDF
+// computes them at construction (sequential FP32 sums in row-major order,
+// before any offset is applied, HDRI.hpp:107-128), then the tables.
+HdriSampler buildHdriSampler(const float* rgb, int width, int height);
+
+// The lookup shared by host and device: binarySearch(cdf, r, L) for r in [0, 1).
+EL_HD inline int hdriLookup(const float* cdf, const int32_t* guide, const int32_t* map2, int log2G, float r) {
+    int c = (int)(r * (float)(1 << log2G));
+    int lo = guide[c], hi = guide[c + 1];
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (cdf[mid] <= r)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return cdf[lo] == r ? map2[2 * lo + 1] : map2[2 * lo];
+}
 
 }  // namespace eleven

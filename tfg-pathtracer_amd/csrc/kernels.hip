@@ -6,8 +6,15 @@
 //                  (kernel.cu:369-481) with per-sample launches
 //                  (kernel.cu:675-683)
 //   k_wf_*         wavefront: camera -> extend -> shade -> shadow x2 ->
-//                  finalize, with per-bounce ray queues compacted by wave
-//                  ballot + one atomic per wave
+//                  finalize over per-bounce ray queues
+//
+// Queues and tallies avoid same-address atomics across the chip: MI355X has 8
+// XCDs with private L2s, and workgroups are dispatched to them round-robin,
+// so blockIdx % 8 names the XCD.  Every queue is split into 8 segments, one
+// per XCD; a block appends all its entries of one loop iteration with a single
+// atomic on its XCD's counter (ballot + LDS prefix), and consumers address the
+// 8 segments as one dense index range.  Tallies are reduced per block and
+// added to per-XCD copies of the counters.
 //
 // All traversal kernels are grid-stride loops over a grid no larger than the
 // device's resident-thread capacity, so the per-thread global spill area of
@@ -20,28 +27,95 @@
 namespace eleven {
 namespace dev {
 
+constexpr int kMaxWaves = kMaxBlock / 64;
+
 __device__ __forceinline__ uint32_t waveSumU32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-__device__ __forceinline__ void flushCount(unsigned long long* dst, uint32_t v) {
-    uint32_t s = waveSumU32(v);
-    if (__lane_id() == 0 && s) atomicAdd(dst, (unsigned long long)s);
+__device__ __forceinline__ uint32_t xcdOf() { return blockIdx.x % kXcds; }
+
+// Block-wide sum of N per-thread tallies, one atomic per tally per block on
+// the block's XCD copy of the counters.  Every thread of the block must call.
+template <int N>
+__device__ __forceinline__ void blockFlush(unsigned long long* counters, const int (&slot)[N],
+                                          const uint32_t (&v)[N]) {
+    __shared__ uint32_t red[kMaxWaves][N];
+    const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        uint32_t s = waveSumU32(v[k]);
+        if (__lane_id() == 0) red[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        unsigned long long s = 0;
+        for (int w = 0; w < nw; w++) s += red[w][threadIdx.x];
+        if (s) atomicAdd(&counters[xcdOf() * kStatSlots + slot[threadIdx.x]], s);
+    }
 }
 
-// Wave-wide slot allocation: one atomic per wave, lanes get consecutive slots.
-__device__ __forceinline__ int waveAlloc(uint32_t* counter, bool want) {
-    unsigned long long mask = __ballot(want);
-    if (mask == 0) return -1;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (!want) return -1;
-    return (int)(base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)));
+// Dense index i over an XCD-segmented queue -> entry index.
+struct QueueMap {
+    uint32_t pre[kXcds + 1];
+    __device__ __forceinline__ void load(const uint32_t* cnt) {
+        pre[0] = 0;
+#pragma unroll
+        for (int k = 0; k < kXcds; k++) pre[k + 1] = pre[k] + cnt[k];
+    }
+    __device__ __forceinline__ uint32_t total() const { return pre[kXcds]; }
+    __device__ __forceinline__ uint64_t entry(uint32_t i, uint32_t segSize) const {
+        // selects only: a dynamic index into pre[] would put it in scratch
+        uint32_t s = 0, base = 0;
+#pragma unroll
+        for (int k = 1; k < kXcds; k++) {
+            bool past = i >= pre[k];
+            s = past ? (uint32_t)k : s;
+            base = past ? pre[k] : base;
+        }
+        return (uint64_t)s * segSize + (i - base);
+    }
+};
+
+// Appends up to three entries per thread (one per queue) with one atomic per
+// queue per block.  Returns entry indices (segment-relative index checked
+// against segSize; ~0 if not wanted or if the segment would overflow).
+// Every thread of the block must call.
+__device__ __forceinline__ void blockAlloc3(const bool (&want)[3], uint32_t* const (&cnt)[3], uint32_t segSize,
+                                           uint64_t (&out)[3], uint32_t& overflow) {
+    __shared__ uint32_t s_cnt[kMaxWaves][3];
+    const int wave = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6, lane = __lane_id();
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t pos[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        unsigned long long m = __ballot(want[k]);
+        pos[k] = (uint32_t)__popcll(m & lt);
+        if (lane == 0) s_cnt[wave][k] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int k = threadIdx.x;
+        uint32_t total = 0;
+        for (int w = 0; w < nw; w++) total += s_cnt[w][k];
+        uint32_t base = total ? atomicAdd(cnt[k] + xcdOf(), total) : 0u;
+        for (int w = 0; w < nw; w++) {
+            uint32_t c = s_cnt[w][k];
+            s_cnt[w][k] = base;
+            base += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        uint32_t idx = s_cnt[wave][k] + pos[k];
+        bool ok = want[k] && idx < segSize;
+        if (want[k] && !ok) overflow = 1;
+        out[k] = ok ? (uint64_t)xcdOf() * segSize + idx : ~0ull;
+    }
+    __syncthreads();   // s_cnt is reused by the next call
 }
 
 __device__ __forceinline__ TravStack makeStack(const DevScene& s, int* gstack, uint32_t gthreads) {
@@ -60,7 +134,7 @@ __device__ __forceinline__ TravStack makeStack(const DevScene& s, int* gstack, u
 
 // ------------------------------------------------------------------ trace --
 template <int W, bool ANY, bool COUNT>
-__global__ void k_trace(DevScene s, const float* __restrict__ rays, const float* __restrict__ tmax, uint32_t n,
+__global__ __launch_bounds__(kMaxBlock) void k_trace(DevScene s, const float* __restrict__ rays, const float* __restrict__ tmax, uint32_t n,
                         int32_t* __restrict__ outTri, float* __restrict__ outT, float* __restrict__ outUV,
                         uint8_t* __restrict__ outOcc, int* gstack, uint32_t gthreads,
                         unsigned long long* counters) {
@@ -84,11 +158,9 @@ __global__ void k_trace(DevScene s, const float* __restrict__ rays, const float*
             }
         }
     }
-    if (COUNT) {
-        flushCount(&counters[0], cnt.nodes);
-        flushCount(&counters[1], cnt.tris);
-    }
-    flushCount(&counters[2], overflow);
+    const int slots[3] = {kStatNodes, kStatTris, kStatOverflow};
+    const uint32_t vals[3] = {cnt.nodes, cnt.tris, overflow};
+    blockFlush<3>(counters, slots, vals);
 }
 
 // --------------------------------------------------------- shared shading --
@@ -106,7 +178,7 @@ struct VertexOut {
 
 // One path vertex: DisneySample, DisneyEval/Pdf, HDRI and point-light NEE
 // terms and the balance-heuristic combine of shade() (kernel.cu:339-358),
-// with the shadow queries left to the caller.
+// with the shadow queries left to the caller.  r = the bounce's Philox block.
 __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, const SurfaceHit& sh,
                                                  const HitData& hd, const uint32_t* r) {
     VertexOut o;
@@ -115,7 +187,7 @@ __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, 
     float bpdf = disneyPdf(d, hd, bdir);
     float r1 = u01(r[3]);
 
-    EnvSample es = envSample(s, r[3], u01(r[4]));
+    EnvSample es = envSample(s, r1);
     v3 H = disneyEval(d, hd, es.dir) * fabsf(dot(es.dir, hd.normal)) * es.value / es.pdf;
     float hpdf = es.pdf;
     o.hOrig = sh.position + es.dir * 0.001f;
@@ -156,7 +228,7 @@ __device__ __forceinline__ v3 clamp3(v3 a) {
 
 // ------------------------------------------------------------------ plain --
 template <int W>
-__global__ void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(kMaxBlock) void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, uint32_t* __restrict__ counts,
                         float4* __restrict__ aov, int* gstack, uint32_t gthreads,
                         unsigned long long* counters) {
     TravStack st = makeStack(s, gstack, gthreads);
@@ -172,10 +244,10 @@ __global__ void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, ui
         if (x >= W_ || y >= H_) continue;
         uint32_t idx = W_ * (H_ - y - 1) + x;
         PathRng rng{a.k0, a.k1, idx, a.sample, a.scene};
-        uint32_t c[5];
-        rng.dims(0, 5, c);
+        uint4 c = rng.block(0);
+        uint32_t c5 = rng.block(1).x;
         v3 o, d;
-        cameraRay(s.cam, (int)x, (int)y, u01(c[0]), u01(c[1]), u01(c[2]), u01(c[3]), u01(c[4]), o, d);
+        cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
         v3 L = mk(0, 0, 0), T = mk(1, 1, 1);
         v3 an = mk(0, 0, 0), at = mk(0, 0, 0), ab = mk(0, 0, 0);
         int i;
@@ -190,8 +262,8 @@ __global__ void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, ui
             }
             SurfaceHit sh = surfaceAt(s, h.tri, h.t, h.u, h.v, o, d);
             HitData hd = hitData(s, sh);
-            uint32_t r[5];
-            rng.dims(5 + 5 * i, 5, r);
+            uint4 rb = rng.block(2 + i);
+            uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
             VertexOut vo = shadeVertex(s, d, T, sh, hd, r);
             rays++;
             TraceResult hs = traceRay<W, true, false>(s, vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, vo.hDir.x, vo.hDir.y,
@@ -229,10 +301,9 @@ __global__ void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, ui
             }
         }
     }
-    flushCount(&counters[0], rays);
-    flushCount(&counters[1], verts);
-    flushCount(&counters[2], paths);
-    flushCount(&counters[3], overflow);
+    const int slots[4] = {kStatRays, kStatVerts, kStatPaths, kStatOverflow};
+    const uint32_t vals[4] = {rays, verts, paths, overflow};
+    blockFlush<4>(counters, slots, vals);
 }
 
 // -------------------------------------------------------------- wavefront --
@@ -246,113 +317,116 @@ __device__ __forceinline__ bool workPixel(const DevScene& s, uint32_t p, uint32_
     return x < s.cam.xRes && y < s.cam.yRes;
 }
 
-__global__ void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
+// Camera rays go to queue 0 at index = path slot (no compaction); padding
+// slots get a zero direction, which the extend kernel reports as dead.
+__global__ __launch_bounds__(kMaxBlock) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
     uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t* cnt = q.counters + 0;   // bounce 0 reads queue 0
-    for (uint32_t b = blockIdx.x * blockDim.x; b < a.genCount; b += stride) {
-        uint32_t slot = b + threadIdx.x;
-        bool valid = slot < a.genCount;
-        uint32_t x = 0, y = 0, idx = 0, sample = 0;
-        if (valid) {
-            uint64_t w = a.genStart + slot;
-            uint32_t p = (uint32_t)(w % a.npad);
-            sample = a.sample + (uint32_t)(w / a.npad);
-            valid = workPixel(s, p, x, y);
-            idx = s.cam.xRes * (s.cam.yRes - y - 1) + x;
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.genCount; slot += stride) {
+        uint64_t w = a.genStart + slot;
+        uint32_t p = (uint32_t)(w % a.npad);
+        uint32_t sample = a.sample + (uint32_t)(w / a.npad);
+        uint32_t x, y;
+        bool valid = workPixel(s, p, x, y);
+        uint32_t idx = valid ? s.cam.xRes * (s.cam.yRes - y - 1) + x : 0xFFFFFFFFu;
+        q.pathT[slot] = make_float4(1, 1, 1, 0);
+        q.pathL[slot] = make_float4(0, 0, 0, valid ? 1.f : 0.f);   // w: path is live
+        q.pathPix[slot] = idx;
+        q.pathSample[slot] = sample;
+        if (q.pathAov) {
+            q.pathAov[3 * (uint64_t)slot + 0] = make_float4(0, 0, 0, 0);
+            q.pathAov[3 * (uint64_t)slot + 1] = make_float4(0, 0, 0, 0);
+            q.pathAov[3 * (uint64_t)slot + 2] = make_float4(0, 0, 0, 0);
         }
-        if (slot < a.genCount) {
-            q.pathT[slot] = make_float4(1, 1, 1, 0);
-            q.pathL[slot] = make_float4(0, 0, 0, valid ? 1.f : 0.f);   // w: path is live
-            q.pathPix[slot] = valid ? idx : 0xFFFFFFFFu;
-            q.pathSample[slot] = sample;
-            if (q.pathAov) {
-                q.pathAov[3 * (uint64_t)slot + 0] = make_float4(0, 0, 0, 0);
-                q.pathAov[3 * (uint64_t)slot + 1] = make_float4(0, 0, 0, 0);
-                q.pathAov[3 * (uint64_t)slot + 2] = make_float4(0, 0, 0, 0);
-            }
-        }
-        int qi = waveAlloc(cnt, valid);
+        v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
         if (valid) {
             PathRng rng{a.k0, a.k1, idx, sample, a.scene};
-            uint32_t c[5];
-            rng.dims(0, 5, c);
-            v3 o, d;
-            cameraRay(s.cam, (int)x, (int)y, u01(c[0]), u01(c[1]), u01(c[2]), u01(c[3]), u01(c[4]), o, d);
-            float4* Q = q.ray[0] + 2 * (uint64_t)qi;
-            Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
-            Q[1] = make_float4(d.x, d.y, d.z, 0.f);
+            uint4 c = rng.block(0);
+            uint32_t c5 = rng.block(1).x;
+            cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
         }
+        float4* Q = q.ray[0] + 2 * (uint64_t)slot;
+        Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
+        Q[1] = make_float4(d.x, d.y, d.z, 0.f);
     }
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) q.counters[kQRay0 + threadIdx.x] = threadIdx.x == 0 ? a.genCount : 0u;
 }
 
 // Also clears the counters the following shade kernel appends to: the next
-// ray queue (`resetIdx`) and both shadow queues.
+// ray queue and both shadow queues.
 template <int W>
-__global__ void k_wf_extend(DevScene s, WfState q, const float4* __restrict__ rayIn, const uint32_t* countIn,
-                            int resetIdx, int countTraversal, unsigned long long* counters) {
+__global__ __launch_bounds__(kMaxBlock) void k_wf_extend(DevScene s, WfState q, const float4* __restrict__ rayIn, const uint32_t* countIn,
+                            uint32_t* resetNext, int countTraversal, unsigned long long* counters) {
     TravStack st = makeStack(s, q.gstack, q.gthreads);
     TraceCounters cnt{0, 0};
-    uint32_t n = *countIn;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        q.counters[resetIdx] = 0;
-        q.counters[2] = 0;
-        q.counters[3] = 0;
+    QueueMap qm;
+    qm.load(countIn);
+    const uint32_t n = qm.total();
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) {
+        resetNext[threadIdx.x] = 0;
+        q.counters[kQShadowH + threadIdx.x] = 0;
+        q.counters[kQShadowP + threadIdx.x] = 0;
     }
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t rays = 0, overflow = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float4 A = rayIn[2 * (uint64_t)i], B = rayIn[2 * (uint64_t)i + 1];
+        uint64_t e = qm.entry(i, q.segSize);
+        float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
         TraceResult h;
-        if (countTraversal)
-            h = traceRay<W, false, true>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
-        else
-            h = traceRay<W, false, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
-        overflow |= st.overflow;
-        q.hits[i] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
-        rays++;
+        if (B.x == 0.f && B.y == 0.f && B.z == 0.f) {
+            h.tri = -2;   // padding slot of the camera queue
+            h.t = h.u = h.v = 0.f;
+        } else {
+            if (countTraversal)
+                h = traceRay<W, false, true>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
+            else
+                h = traceRay<W, false, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
+            overflow |= st.overflow;
+            rays++;
+        }
+        q.hits[e] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
     }
-    flushCount(&counters[0], rays);
-    flushCount(&counters[3], overflow);
-    if (countTraversal) {
-        flushCount(&counters[4], cnt.nodes);
-        flushCount(&counters[5], cnt.tris);
-    }
+    const int slots[4] = {kStatRays, kStatOverflow, kStatNodes, kStatTris};
+    const uint32_t vals[4] = {rays, overflow, cnt.nodes, cnt.tris};
+    blockFlush<4>(counters, slots, vals);
 }
 
-__global__ void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
+__global__ __launch_bounds__(kMaxBlock) void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
                            const uint32_t* countIn, float4* __restrict__ rayOut, uint32_t* countOut,
                            unsigned long long* counters) {
-    uint32_t n = *countIn;
+    QueueMap qm;
+    qm.load(countIn);
+    const uint32_t n = qm.total();
     uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t rays = 0, verts = 0;
+    uint32_t rays = 0, verts = 0, overflow = 0;
     const int bounce = (int)a.bounce;
     const bool last = bounce + 1 >= (int)a.maxDepth;
+    uint32_t* const outCnt[3] = {countOut, q.counters + kQShadowH, q.counters + kQShadowP};
     for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += stride) {
         uint32_t i = b + threadIdx.x;
-        bool valid = i < n;
         bool cont = false, wantH = false, wantP = false;
         VertexOut vo;
         uint32_t slot = 0;
-        if (valid) {
-            float4 A = rayIn[2 * (uint64_t)i], B = rayIn[2 * (uint64_t)i + 1];
-            float4 hr = q.hits[i];
+        if (i < n) {
+            uint64_t e = qm.entry(i, q.segSize);
+            float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
+            float4 hr = q.hits[e];
             slot = __float_as_uint(A.w);
-            v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
-            float4 Tf = q.pathT[slot];
-            v3 T = mk(Tf.x, Tf.y, Tf.z);
             int tri = __float_as_int(hr.x);
-            if (tri < 0) {
-                v3 e = envLookup(s, d) * T;
+            if (tri == -1) {
+                v3 d = mk(B.x, B.y, B.z);
+                float4 Tf = q.pathT[slot];
+                v3 ev = envLookup(s, d) * mk(Tf.x, Tf.y, Tf.z);
                 float4 L = q.pathL[slot];
-                q.pathL[slot] = make_float4(L.x + e.x, L.y + e.y, L.z + e.z, L.w);
-            } else {
+                q.pathL[slot] = make_float4(L.x + ev.x, L.y + ev.y, L.z + ev.z, L.w);
+            } else if (tri >= 0) {
+                v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
+                float4 Tf = q.pathT[slot];
                 SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
                 HitData hd = hitData(s, sh);
-                uint32_t pix = q.pathPix[slot];
-                PathRng rng{a.k0, a.k1, pix, q.pathSample[slot], a.scene};
-                uint32_t r[5];
-                rng.dims(5 + 5 * bounce, 5, r);
-                vo = shadeVertex(s, d, T, sh, hd, r);
+                PathRng rng{a.k0, a.k1, q.pathPix[slot], q.pathSample[slot], a.scene};
+                uint4 rb = rng.block(2 + bounce);
+                uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
+                vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
                 float4 L = q.pathL[slot];
                 q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
                 q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
@@ -370,56 +444,62 @@ __global__ void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __
                 cont = !last;
             }
         }
-        int qi = waveAlloc(countOut, cont);
-        if (cont) {
-            float4* Q = rayOut + 2 * (uint64_t)qi;
+        const bool want[3] = {cont, wantH, wantP};
+        uint64_t at[3];
+        blockAlloc3(want, outCnt, q.segSize, at, overflow);
+        if (at[0] != ~0ull) {
+            float4* Q = rayOut + 2 * at[0];
             Q[0] = make_float4(vo.nextO.x, vo.nextO.y, vo.nextO.z, __uint_as_float(slot));
             Q[1] = make_float4(vo.nextD.x, vo.nextD.y, vo.nextD.z, 0.f);
         }
-        int hi = waveAlloc(q.counters + 2, wantH);
-        if (wantH) {
-            float4* S = q.shadowH + 3 * (uint64_t)hi;
+        if (at[1] != ~0ull) {
+            float4* S = q.shadowH + 3 * at[1];
             S[0] = make_float4(vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, __uint_as_float(slot));
             S[1] = make_float4(vo.hDir.x, vo.hDir.y, vo.hDir.z, __int_as_float(0x7f800000));
             S[2] = make_float4(vo.contribH.x, vo.contribH.y, vo.contribH.z, 0);
         }
-        int pi = waveAlloc(q.counters + 3, wantP);
-        if (wantP) {
-            float4* S = q.shadowP + 3 * (uint64_t)pi;
+        if (at[2] != ~0ull) {
+            float4* S = q.shadowP + 3 * at[2];
             S[0] = make_float4(vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, __uint_as_float(slot));
             S[1] = make_float4(vo.pDir.x, vo.pDir.y, vo.pDir.z, vo.pTmax);
             S[2] = make_float4(vo.contribP.x, vo.contribP.y, vo.contribP.z, 0);
         }
     }
-    flushCount(&counters[0], rays);
-    flushCount(&counters[1], verts);
+    const int slots[3] = {kStatRays, kStatVerts, kStatOverflow};
+    const uint32_t vals[3] = {rays, verts, overflow};
+    blockFlush<3>(counters, slots, vals);
 }
 
 template <int W>
-__global__ void k_wf_shadow(DevScene s, WfState q, const float4* __restrict__ sh, const uint32_t* countIn,
+__global__ __launch_bounds__(kMaxBlock) void k_wf_shadow(DevScene s, WfState q, const float4* __restrict__ sh, const uint32_t* countIn,
                             unsigned long long* counters) {
     TravStack st = makeStack(s, q.gstack, q.gthreads);
     TraceCounters cnt{0, 0};
-    uint32_t n = *countIn;
+    QueueMap qm;
+    qm.load(countIn);
+    const uint32_t n = qm.total();
     uint32_t stride = gridDim.x * blockDim.x;
     uint32_t overflow = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        float4 A = sh[3 * (uint64_t)i], B = sh[3 * (uint64_t)i + 1];
+        uint64_t e = qm.entry(i, q.segSize);
+        float4 A = sh[3 * e], B = sh[3 * e + 1];
         TraceResult h = traceRay<W, true, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, B.w, st, cnt);
         overflow |= st.overflow;
         if (h.tri < 0) {
-            float4 C = sh[3 * (uint64_t)i + 2];
+            float4 C = sh[3 * e + 2];
             uint32_t slot = __float_as_uint(A.w);
             float4 L = q.pathL[slot];
             q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
         }
     }
-    flushCount(&counters[3], overflow);
+    const int slots[1] = {kStatOverflow};
+    const uint32_t vals[1] = {overflow};
+    blockFlush<1>(counters, slots, vals);
 }
 
 // One thread per distinct pixel of the generation; slots j, j+npad, ... hold
 // the same pixel, so accumulation needs no atomics.
-__global__ void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __restrict__ accum,
+__global__ __launch_bounds__(kMaxBlock) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __restrict__ accum,
                               uint32_t* __restrict__ counts, float4* __restrict__ aov,
                               unsigned long long* counters) {
     uint32_t stride = gridDim.x * blockDim.x;
@@ -443,12 +523,9 @@ __global__ void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __res
             }
         }
     }
-    flushCount(&counters[2], paths);
-}
-
-__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
-    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
-        p[i] = v;
+    const int slots[1] = {kStatPaths};
+    const uint32_t vals[1] = {paths};
+    blockFlush<1>(counters, slots, vals);
 }
 
 }  // namespace dev
@@ -510,15 +587,15 @@ hipError_t launchWfCamera(const LaunchCfg& c, const DevScene& s, const RenderArg
 }
 
 hipError_t launchWfExtend(const LaunchCfg& c, const DevScene& s, const WfState& q, const float4* rayIn,
-                          const uint32_t* countIn, int resetIdx, uint32_t maxCount, bool countTraversal,
+                          const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount, bool countTraversal,
                           unsigned long long* counters) {
     uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
     if (s.width == 4)
-        hipLaunchKernelGGL(k_wf_extend<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetIdx,
+        hipLaunchKernelGGL(k_wf_extend<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetNext,
                            countTraversal ? 1 : 0, counters);
     else
-        hipLaunchKernelGGL(k_wf_extend<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetIdx,
+        hipLaunchKernelGGL(k_wf_extend<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetNext,
                            countTraversal ? 1 : 0, counters);
     return hipGetLastError();
 }
@@ -551,14 +628,6 @@ hipError_t launchWfFinalize(const LaunchCfg& c, const DevScene& s, const RenderA
     uint32_t grid = gridFor(lim, c.block, c.maxThreads / c.block);
     hipLaunchKernelGGL(k_wf_finalize, dim3(grid), dim3(c.block), 0, c.stream, s, a, q, accum, counts, aov,
                        counters);
-    return hipGetLastError();
-}
-
-hipError_t launchFillU32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n) {
-    uint32_t grid = (uint32_t)((n + 255) / 256);
-    if (grid > 4096) grid = 4096;
-    if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, st, p, v, n);
     return hipGetLastError();
 }
 

@@ -9,6 +9,20 @@ namespace dev {
 struct DevScene;
 }
 
+// MI355X dispatches workgroups to its 8 XCDs round-robin; queues and tallies
+// keep one segment / copy per XCD (see kernels.hip).
+constexpr int kXcds = 8;
+
+// Largest workgroup the kernels are compiled for (launch bounds).
+constexpr int kMaxBlock = 256;
+
+// Tally slots (per XCD copy): the host sums the kXcds copies.
+constexpr int kStatSlots = 8;
+constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5;
+
+// Queue counters: kXcds per queue.
+constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadowH = 2 * kXcds, kQShadowP = 3 * kXcds, kQCounters = 4 * kXcds;
+
 struct RenderArgs {
     uint32_t k0, k1;        // RNG key (seed)
     uint32_t sample;        // first sample index of this launch / generation
@@ -30,7 +44,8 @@ struct WfState {
     uint32_t* pathPix;      // output pixel index per path slot
     uint32_t* pathSample;   // sample index per path slot
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
-    uint32_t* counters;     // [0],[1] ray queues, [2] HDRI shadows, [3] point shadows
+    uint32_t* counters;     // kQCounters: per-XCD segment counts of the four queues
+    uint32_t segSize;       // entries per XCD segment of a compacted queue
     int* gstack;
     uint32_t gthreads;
 };
@@ -49,7 +64,7 @@ hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderA
                        uint32_t* counts, float4* aov, unsigned long long* counters);
 hipError_t launchWfCamera(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q);
 hipError_t launchWfExtend(const LaunchCfg& c, const dev::DevScene& s, const WfState& q, const float4* rayIn,
-                          const uint32_t* countIn, int resetIdx, uint32_t maxCount, bool countTraversal,
+                          const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount, bool countTraversal,
                           unsigned long long* counters);
 hipError_t launchWfShade(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q,
                          const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
@@ -58,6 +73,5 @@ hipError_t launchWfShadow(const LaunchCfg& c, const dev::DevScene& s, const WfSt
                           const uint32_t* countIn, uint32_t maxCount, unsigned long long* counters);
 hipError_t launchWfFinalize(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q,
                             float4* accum, uint32_t* counts, float4* aov, unsigned long long* counters);
-hipError_t launchFillU32(hipStream_t st, uint32_t* p, uint32_t v, uint64_t n);
 
 }  // namespace eleven

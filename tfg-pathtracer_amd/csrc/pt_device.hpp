@@ -15,14 +15,15 @@
 //     t < dist - 0.001 from the offset origin (the reference compares the
 //     closest hit's smooth-shading position, kernel.cu:193-197; the two differ
 //     only for an occluder within the shading offset of the light).
-//   * The HDRI pixel is drawn from an alias table of the reference search's
-//     effective distribution (bin from the 32-bit shade-r1 draw, acceptance
-//     from an extra draw); the point light uses the same r1 as the reference.
+//   * The HDRI pixel is binarySearch(cdf, r1) bit for bit, found through a
+//     guide table (hdri_table.hpp); the point light uses the same r1.
 //   * acos arguments are clamped to [-1, 1] for the environment lookup.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "hdri_table.hpp"
 
 namespace eleven {
 namespace dev {
@@ -78,26 +79,15 @@ __device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1
 }
 __device__ __forceinline__ float u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
 
-// Dimension layout of one path: dims 0..4 camera (r1..r5), then per bounce
-// b the five dims 5+5b .. 9+5b: DisneySample r1,r2,r3; shade r1; alias accept.
+// Random numbers of one path, one Philox block per use:
+//   block 0      camera r1..r4          block 1 .x   camera r5
+//   block 2 + b  bounce b: DisneySample r1, r2, r3 (.x .y .z), shade r1 (.w)
+// (shade's r2 and r3 are never read on the reference's HDRI-IS path,
+// kernel.cu:234-257, so they are not drawn).
 struct PathRng {
     uint32_t k0, k1, pixel, sample, scene;
     __device__ __forceinline__ uint4 block(uint32_t b) const {
         return philox4x32_10(make_uint4(pixel, sample, b, scene), k0, k1);
-    }
-    // Draw dims [d, d+n) (n <= 5) into out[].
-    __device__ __forceinline__ void dims(uint32_t d, int n, uint32_t* out) const {
-        uint32_t b = d >> 2;
-        uint4 q = block(b);
-        uint32_t w[8] = {q.x, q.y, q.z, q.w, 0, 0, 0, 0};
-        if (((d & 3) + n) > 4) {
-            uint4 q2 = block(b + 1);
-            w[4] = q2.x;
-            w[5] = q2.y;
-            w[6] = q2.z;
-            w[7] = q2.w;
-        }
-        for (int i = 0; i < n; i++) out[i] = w[(d & 3) + i];
     }
 };
 
@@ -120,7 +110,8 @@ struct DevMaterial {
 struct DevCamera {
     uint32_t xRes, yRes;
     float focalLength, sensorWidth, sensorHeight, aperture, focusDistance;
-    float rot[3], pos[3];
+    float cosRot[3], sinRot[3];   // cos/sin of the Euler angles in radians, per camera
+    float pos[3];
     int32_t bokeh;
 };
 
@@ -137,11 +128,12 @@ struct DevScene {
     const DevTex* texs;
     const float* texels;           // pool of all textures (RGB FP32)
     const float* lights;           // 6 floats per light: position, radiance
-    const float* envProb;          // alias acceptance per bin
-    const int32_t* envAlias;       // alias target per bin
+    const float* envCdf;           // HDRI CDF (L + 1)
+    const int32_t* envGuide;       // guide table (2^envLog2G + 1)
+    const int32_t* envMap;         // per interval: open-interval and tie results
     DevTex env;                    // env texture (texels in the pool)
     float envRadianceSum;
-    int32_t envLen;
+    int32_t envLog2G;
     int32_t nLights;
     int32_t nTris;
     int32_t width;                 // BVH width (4 or 8)
@@ -343,12 +335,14 @@ __device__ __forceinline__ void cameraRay(const DevCamera& c, int x, int y, floa
     float ry = (float)((1.0 / (float)c.yRes) * (r2 - 0.5) * c.sensorHeight);
     float SPx = odx + rx, SPy = ody + ry;
     float SPz = c.pos[2] + c.focalLength;
-    v3 rot = mk(c.rot[0], c.rot[1], c.rot[2]);
-    rot = rot * (float)(EL_PI / 180.0);
+    // XYZ Euler rotation; the per-camera cos/sin are evaluated once on the host
+    // (the reference evaluates the same six values for every ray)
+    const float cx = c.cosRot[0], sx = c.sinRot[0], cy = c.cosRot[1], sy = c.sinRot[1];
+    const float cz = c.cosRot[2], sz = c.sinRot[2];
     v3 dir = mk(SPx, SPy, SPz) - mk(c.pos[0], c.pos[1], c.pos[2]);
-    v3 dX = mk(dir.x, dir.y * cosf(rot.x) - dir.z * sinf(rot.x), dir.y * sinf(rot.x) + dir.z * cosf(rot.x));
-    v3 dY = mk(dX.x * cosf(rot.y) + dX.z * sinf(rot.y), dX.y, dX.z * cosf(rot.y) - dX.x * sinf(rot.y));
-    v3 dZ = mk(dY.x * cosf(rot.z) - dY.y * sinf(rot.z), dY.x * sinf(rot.z) + dY.y * cosf(rot.z), dY.z);
+    v3 dX = mk(dir.x, dir.y * cx - dir.z * sx, dir.y * sx + dir.z * cx);
+    v3 dY = mk(dX.x * cy + dX.z * sy, dX.y, dX.z * cy - dX.x * sy);
+    v3 dZ = mk(dY.x * cz - dY.y * sz, dY.x * sz + dY.y * cz, dY.z);
     o = mk(c.pos[0], c.pos[1], c.pos[2]);
     d = normalized(dZ);
     if (c.bokeh) {
@@ -443,9 +437,9 @@ struct EnvSample {
     v3 dir, value;
     float pdf;
 };
-__device__ __forceinline__ EnvSample envSample(const DevScene& s, uint32_t ubin, float accept) {
-    uint32_t bin = (uint32_t)(((uint64_t)ubin * (uint64_t)s.envLen) >> 32);
-    int k = accept < s.envProb[bin] ? (int)bin : s.envAlias[bin];
+__device__ __forceinline__ EnvSample envSample(const DevScene& s, float r1) {
+    // HDRI::sample(r1): exactly binarySearch(cdf, r1, W*H)
+    int k = hdriLookup(s.envCdf, s.envGuide, s.envMap, s.envLog2G, r1);
     const DevTex& t = s.env;
     int x = k % t.w, y = k / t.w;
     float nu = x / (float)t.w, nv = y / (float)t.h;

@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/eleven.h"
+#include "hdri_table.hpp"
 #include "scene.hpp"
 
 using namespace eleven;
@@ -174,6 +175,20 @@ int el_scene_hdri_cdf(const el_scene* s, const float** cdf, float* radianceSum) 
 }
 
 void el_scene_free(el_scene* s) { delete s; }
+
+int el_hdri_lookup(const el_texture* hdri, const float* r, uint32_t n, int32_t* pixel) {
+    if (!hdri || !hdri->data || hdri->width <= 0 || hdri->height <= 0 || (n && (!r || !pixel))) return EL_ERR_ARG;
+    try {
+        HdriSampler s = buildHdriSampler(hdri->data, hdri->width, hdri->height);
+        for (uint32_t i = 0; i < n; i++) {
+            if (!(r[i] >= 0.f && r[i] < 1.f)) return EL_ERR_RANGE;
+            pixel[i] = hdriLookup(s.cdf.data(), s.guide.data(), s.map2.data(), s.log2G, r[i]);
+        }
+    } catch (const std::bad_alloc&) {
+        return EL_ERR_NOMEM;
+    }
+    return EL_OK;
+}
 
 const char* el_scene_last_error(void) { return g_sceneError.c_str(); }
 

@@ -96,6 +96,7 @@ _SIGS = {
     "el_scene_view_get": (C.c_int, [_P, C.POINTER(el_scene_view)]),
     "el_scene_hdri_cdf": (C.c_int, [_P, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_float)]),
     "el_scene_free": (None, [_P]),
+    "el_hdri_lookup": (C.c_int, [C.POINTER(el_texture), _P, C.c_uint32, _P]),
     "el_scene_last_error": (C.c_char_p, []),
     "el_create": (C.c_int, [C.POINTER(_P), C.c_int]),
     "el_destroy": (None, [_P]),
@@ -139,6 +140,17 @@ def lib():
 
 def _ptr(a: np.ndarray, ctype=C.c_float):
     return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def hdri_lookup(rgb: np.ndarray, r: np.ndarray) -> np.ndarray:
+    """Environment pixel index the renderer draws for each uniform r (host evaluation)."""
+    t = make_texture(rgb)
+    rr = np.ascontiguousarray(r, np.float32)
+    out = np.zeros(rr.size, np.int32)
+    rc = lib().el_hdri_lookup(C.byref(t), rr.ctypes.data, rr.size, out.ctypes.data)
+    if rc != EL_OK:
+        raise ElevenError(rc, "el_hdri_lookup failed")
+    return out
 
 
 def device_count() -> int:

@@ -318,9 +318,12 @@ def heightfield(out_dir, cells=(1024, 512), extent=(51.2, 25.6), res=(1920, 1080
     write_hdr(out / "HDRI" / "sky.hdr", sky_hdr(*sky))
     lp = [(float(rng.uniform(-extent[0], extent[0])), float(rng.uniform(6, 12)),
            float(rng.uniform(-extent[1], extent[1]))) for _ in range(lights)]
-    cam_pos = (0.0, 22.0, -60.0)
-    pitch = math.degrees(math.atan2(22.0, 60.0))
-    focus = math.sqrt(22.0 ** 2 + 60.0 ** 2)
+    # framed so the terrain fills the 16:9 view: the top row of pixels meets
+    # the far edge, the bottom row the near half of the grid
+    k = extent[0] / 51.2
+    cam_pos = (0.0, 15.0 * k, -38.0 * k)
+    pitch = 29.0
+    focus = math.sqrt(cam_pos[1] ** 2 + cam_pos[2] ** 2)
     camera = {"xRes": res[0], "yRes": res[1], "focalLength": 0.035, "focusDistance": focus, "aperture": aperture,
               "bokeh": bokeh, "position": {"x": cam_pos[0], "y": cam_pos[1], "z": cam_pos[2]},
               "rotation": {"x": pitch, "y": 0.0, "z": 0.0}}
