@@ -83,12 +83,27 @@ def main():
     ap.add_argument("--bvh", type=int, default=0, help="BVH width 4 or 8 (0: library default)")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--queue", type=int, default=0)
+    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--res", default="1920x1080")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--save", default="")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 launches): the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
+
+    if args.copy_calib:
+        import torch
+        n = args.copy_calib * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calib_bytes_read": n * 4, "copy_calib_bytes_written": n * 4, "launches": 3}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is "
@@ -110,7 +125,8 @@ def main():
     st0 = ctx.stats()
     log(f"upload (BVH build + copy) {time.perf_counter() - t0:.2f} s, {scene.tri_count} tris")
     mode = eleven.MODE_WAVEFRONT if args.mode == "wavefront" else eleven.MODE_PLAIN
-    kw = dict(depth=args.depth, mode=mode, bvh_width=args.bvh, block=args.block, queue_cap=args.queue)
+    kw = dict(depth=args.depth, mode=mode, bvh_width=args.bvh, block=args.block, queue_cap=args.queue,
+              refill=args.refill)
 
     for w in range(args.warmup):
         ctx.clear()

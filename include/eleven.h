@@ -112,10 +112,13 @@ typedef struct {
     uint32_t max_depth;    /* MAXBOUNCES (Definitions.h:9); 0 -> 5          */
     uint64_t seed;         /* RNG key                                       */
     int32_t  mode;         /* EL_MODE_*                                     */
-    uint32_t block;        /* threads per block: 64..256 step 64, 0 -> 256   */
+    uint32_t block;        /* traversal threads per block: 64..256 step 64,
+                              0 -> tuned default                            */
     uint32_t queue_cap;    /* wavefront paths in flight, 0 -> default       */
     uint32_t bvh_width;    /* 4 or 8, 0 -> current                          */
     uint32_t flags;        /* EL_FLAG_*                                     */
+    uint32_t refill;       /* wavefront traversal: idle lanes (1..64) that
+                              trigger an in-wave ray refill, 0 -> default   */
 } el_render_params;
 
 typedef struct {

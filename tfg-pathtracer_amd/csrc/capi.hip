@@ -71,7 +71,8 @@ struct el_ctx {
     std::string err;
     int cus = 0;
     uint32_t maxThreads = 0;
-    uint32_t block = 256;
+    uint32_t block = kDefaultBlock;
+    uint32_t refill = kDefaultRefill;
 
     bool hasScene = false;
     uint32_t ntri = 0;
@@ -127,6 +128,8 @@ static LaunchCfg cfgOf(el_ctx* c) {
     LaunchCfg l;
     l.stream = c->stream;
     l.block = c->block;
+    l.shadeBlock = 256;
+    l.refill = c->refill;
     l.maxThreads = c->maxThreads;
     l.gstack = c->dGstack.as<int>();
     return l;
@@ -432,7 +435,9 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         int rc = uploadBvh(c, (int)p->bvh_width);
         if (rc) return rc;
     }
-    c->block = p->block ? p->block : 256;
+    if (p->refill > 64) return fail(c, EL_ERR_ARG, "refill must be 0..64");
+    c->block = p->block ? p->block : kDefaultBlock;
+    c->refill = p->refill ? p->refill : kDefaultRefill;
     bool aov = (p->flags & EL_FLAG_AOV) != 0;
     size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
     if (aov && !c->dAov.p) {
@@ -467,7 +472,16 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         const uint32_t tilesX = (c->cam.x_res + 7) / 8, tilesY = (c->cam.y_res + 7) / 8;
         const uint64_t npad = (uint64_t)tilesX * tilesY * 64;
         uint64_t total = (uint64_t)p->spp * npad;
-        uint64_t Q = p->queue_cap ? p->queue_cap : (1u << 22);
+        // Default: as many paths in flight as a quarter of the free device
+        // memory holds (~216 B of queue and path state per path), so a
+        // 1080p x 64 spp frame runs as a single generation.
+        uint64_t Q = p->queue_cap;
+        if (!Q) {
+            size_t freeB = 0, totalB = 0;
+            EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
+            const uint64_t perPath = 6 * 32 + 40 + (aov ? 48 : 0);
+            Q = std::max<uint64_t>(1u << 20, freeB / 4 / perPath);
+        }
         if (Q >= npad)
             Q = (Q / npad) * npad;
         else
@@ -475,8 +489,9 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         if (Q > total) Q = total;
         if (Q > 0x7fffffffull) return fail(c, EL_ERR_RANGE, "queue capacity above 2^31");
         // Entries one XCD's blocks can append in one launch: at most
-        // ceil(Q/8) + maxThreads/8 + block (grid-stride over <= maxThreads).
-        const uint64_t segSize = (Q + kXcds - 1) / kXcds + lc.maxThreads / kXcds + lc.block;
+        // ceil(Q/8) + maxThreads/8 + block (grid-stride over <= maxThreads,
+        // block <= kMaxBlock).
+        const uint64_t segSize = (Q + kXcds - 1) / kXcds + lc.maxThreads / kXcds + kMaxBlock;
         const uint64_t qcap = std::max<uint64_t>(Q, kXcds * segSize);
         int rc = ensureWf(c, Q, qcap, aov);
         if (rc) return rc;

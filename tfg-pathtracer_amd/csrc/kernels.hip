@@ -29,6 +29,20 @@ namespace dev {
 
 constexpr int kMaxWaves = kMaxBlock / 64;
 
+// Minimum waves per SIMD the register allocator must leave room for, per
+// kernel family (1 = no constraint).  Traversal is latency-bound and gains
+// from occupancy; the plain megakernel trades spills against occupancy.
+#ifndef EL_WPE_TRAV
+#define EL_WPE_TRAV 8
+#endif
+#ifndef EL_WPE_SHADE
+#define EL_WPE_SHADE 1
+#endif
+#ifndef EL_WPE_PLAIN
+#define EL_WPE_PLAIN 4
+#endif
+#define EL_KERNEL(wpe) __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(wpe)))
+
 __device__ __forceinline__ uint32_t waveSumU32(uint32_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -133,20 +147,41 @@ __device__ __forceinline__ TravStack makeStack(const DevScene& s, int* gstack, u
 }
 
 // ------------------------------------------------------------------ trace --
+// Items [begin, end) of the calling wave: n items split evenly over the
+// grid's waves (block sizes are multiples of 64).
+// Items [begin, end) of the calling wave: n items in chunks of a multiple of
+// 64 over the grid's waves (block sizes are multiples of 64), so a small
+// batch fills few waves completely rather than many waves with one lane.
+__device__ __forceinline__ void waveRange(uint32_t n, uint32_t& begin, uint32_t& end) {
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint64_t chunk = (((uint64_t)n + waves - 1) / waves + 63) & ~63ull;
+    const uint64_t b = chunk * w, e = b + chunk;
+    begin = (uint32_t)(b < n ? b : n);
+    end = (uint32_t)(e < n ? e : n);
+}
+
 template <int W, bool ANY, bool COUNT>
-__global__ __launch_bounds__(kMaxBlock) void k_trace(DevScene s, const float* __restrict__ rays, const float* __restrict__ tmax, uint32_t n,
-                        int32_t* __restrict__ outTri, float* __restrict__ outT, float* __restrict__ outUV,
-                        uint8_t* __restrict__ outOcc, int* gstack, uint32_t gthreads,
-                        unsigned long long* counters) {
+EL_KERNEL(EL_WPE_TRAV) void k_trace(DevScene s, const float* __restrict__ rays, const float* __restrict__ tmax,
+                                    uint32_t n, int32_t* __restrict__ outTri, float* __restrict__ outT,
+                                    float* __restrict__ outUV, uint8_t* __restrict__ outOcc, int* gstack,
+                                    uint32_t gthreads, uint32_t refill, unsigned long long* counters) {
     TravStack st = makeStack(s, gstack, gthreads);
-    uint32_t stride = gridDim.x * blockDim.x;
     TraceCounters cnt{0, 0};
-    uint32_t overflow = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    uint32_t begin, end, traced = 0;
+    waveRange(n, begin, end);
+    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm) {
         const float* r = rays + (uint64_t)i * 6;
-        float tm = ANY ? tmax[i] : __int_as_float(0x7f800000);
-        TraceResult h = traceRay<W, ANY, COUNT>(s, r[0], r[1], r[2], r[3], r[4], r[5], tm, st, cnt);
-        overflow |= st.overflow;
+        ox = r[0];
+        oy = r[1];
+        oz = r[2];
+        dx = r[3];
+        dy = r[4];
+        dz = r[5];
+        tm = ANY ? tmax[i] : __int_as_float(0x7f800000);
+        return true;
+    };
+    auto store = [&](uint32_t i, const TraceResult& h, bool) {
         if (ANY) {
             outOcc[i] = h.tri >= 0 ? 1 : 0;
         } else {
@@ -157,9 +192,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_trace(DevScene s, const float* __
                 outUV[2 * (uint64_t)i + 1] = h.v;
             }
         }
-    }
+    };
+    traceWaveRange<W, ANY, COUNT>(s, st, begin, end, refill, load, store, cnt, traced);
     const int slots[3] = {kStatNodes, kStatTris, kStatOverflow};
-    const uint32_t vals[3] = {cnt.nodes, cnt.tris, overflow};
+    const uint32_t vals[3] = {cnt.nodes, cnt.tris, (uint32_t)st.overflow};
     blockFlush<3>(counters, slots, vals);
 }
 
@@ -228,7 +264,7 @@ __device__ __forceinline__ v3 clamp3(v3 a) {
 
 // ------------------------------------------------------------------ plain --
 template <int W>
-__global__ __launch_bounds__(kMaxBlock) void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, uint32_t* __restrict__ counts,
+EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, uint32_t* __restrict__ counts,
                         float4* __restrict__ aov, int* gstack, uint32_t gthreads,
                         unsigned long long* counters) {
     TravStack st = makeStack(s, gstack, gthreads);
@@ -319,7 +355,7 @@ __device__ __forceinline__ bool workPixel(const DevScene& s, uint32_t p, uint32_
 
 // Camera rays go to queue 0 at index = path slot (no compaction); padding
 // slots get a zero direction, which the extend kernel reports as dead.
-__global__ __launch_bounds__(kMaxBlock) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
+EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.genCount; slot += stride) {
         uint64_t w = a.genStart + slot;
@@ -353,9 +389,10 @@ __global__ __launch_bounds__(kMaxBlock) void k_wf_camera(DevScene s, RenderArgs 
 
 // Also clears the counters the following shade kernel appends to: the next
 // ray queue and both shadow queues.
-template <int W>
-__global__ __launch_bounds__(kMaxBlock) void k_wf_extend(DevScene s, WfState q, const float4* __restrict__ rayIn, const uint32_t* countIn,
-                            uint32_t* resetNext, int countTraversal, unsigned long long* counters) {
+template <int W, bool COUNT>
+EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(DevScene s, WfState q, const float4* __restrict__ rayIn,
+                                        const uint32_t* countIn, uint32_t* resetNext, uint32_t refill,
+                                        unsigned long long* counters) {
     TravStack st = makeStack(s, q.gstack, q.gthreads);
     TraceCounters cnt{0, 0};
     QueueMap qm;
@@ -366,31 +403,31 @@ __global__ __launch_bounds__(kMaxBlock) void k_wf_extend(DevScene s, WfState q, 
         q.counters[kQShadowH + threadIdx.x] = 0;
         q.counters[kQShadowP + threadIdx.x] = 0;
     }
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t rays = 0, overflow = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t e = qm.entry(i, q.segSize);
-        float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
-        TraceResult h;
-        if (B.x == 0.f && B.y == 0.f && B.z == 0.f) {
-            h.tri = -2;   // padding slot of the camera queue
-            h.t = h.u = h.v = 0.f;
-        } else {
-            if (countTraversal)
-                h = traceRay<W, false, true>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
-            else
-                h = traceRay<W, false, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, __int_as_float(0x7f800000), st, cnt);
-            overflow |= st.overflow;
-            rays++;
-        }
-        q.hits[e] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
-    }
+    uint32_t begin, end, rays = 0;
+    waveRange(n, begin, end);
+    const uint32_t segSize = q.segSize;
+    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm) {
+        const uint64_t e = qm.entry(i, segSize);
+        const float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
+        ox = A.x;
+        oy = A.y;
+        oz = A.z;
+        dx = B.x;
+        dy = B.y;
+        dz = B.z;
+        tm = __int_as_float(0x7f800000);
+        return !(B.x == 0.f && B.y == 0.f && B.z == 0.f);   // zero direction: camera-queue padding
+    };
+    auto store = [&](uint32_t i, const TraceResult& h, bool) {
+        q.hits[qm.entry(i, segSize)] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
+    };
+    traceWaveRange<W, false, COUNT>(s, st, begin, end, refill, load, store, cnt, rays);
     const int slots[4] = {kStatRays, kStatOverflow, kStatNodes, kStatTris};
-    const uint32_t vals[4] = {rays, overflow, cnt.nodes, cnt.tris};
+    const uint32_t vals[4] = {rays, (uint32_t)st.overflow, cnt.nodes, cnt.tris};
     blockFlush<4>(counters, slots, vals);
 }
 
-__global__ __launch_bounds__(kMaxBlock) void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
+EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
                            const uint32_t* countIn, float4* __restrict__ rayOut, uint32_t* countOut,
                            unsigned long long* counters) {
     QueueMap qm;
@@ -470,36 +507,50 @@ __global__ __launch_bounds__(kMaxBlock) void k_wf_shade(DevScene s, RenderArgs a
     blockFlush<3>(counters, slots, vals);
 }
 
+// Any-hit test of a shadow queue; an unoccluded entry adds its contribution
+// to its path's radiance (each path has at most one entry per queue, and the
+// two shadow queues are separate launches, so the update needs no atomics).
 template <int W>
-__global__ __launch_bounds__(kMaxBlock) void k_wf_shadow(DevScene s, WfState q, const float4* __restrict__ sh, const uint32_t* countIn,
-                            unsigned long long* counters) {
+EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(DevScene s, WfState q, const float4* __restrict__ sh,
+                                        const uint32_t* countIn, uint32_t refill,
+                                        unsigned long long* counters) {
     TravStack st = makeStack(s, q.gstack, q.gthreads);
     TraceCounters cnt{0, 0};
     QueueMap qm;
     qm.load(countIn);
     const uint32_t n = qm.total();
-    uint32_t stride = gridDim.x * blockDim.x;
-    uint32_t overflow = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        uint64_t e = qm.entry(i, q.segSize);
-        float4 A = sh[3 * e], B = sh[3 * e + 1];
-        TraceResult h = traceRay<W, true, false>(s, A.x, A.y, A.z, B.x, B.y, B.z, B.w, st, cnt);
-        overflow |= st.overflow;
-        if (h.tri < 0) {
-            float4 C = sh[3 * e + 2];
-            uint32_t slot = __float_as_uint(A.w);
-            float4 L = q.pathL[slot];
-            q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
-        }
-    }
+    uint32_t begin, end, traced = 0;
+    waveRange(n, begin, end);
+    const uint32_t segSize = q.segSize;
+    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm) {
+        const uint64_t e = qm.entry(i, segSize);
+        const float4 A = sh[3 * e], B = sh[3 * e + 1];
+        ox = A.x;
+        oy = A.y;
+        oz = A.z;
+        dx = B.x;
+        dy = B.y;
+        dz = B.z;
+        tm = B.w;
+        return true;
+    };
+    auto store = [&](uint32_t i, const TraceResult& h, bool) {
+        if (h.tri >= 0) return;
+        const uint64_t e = qm.entry(i, segSize);
+        const uint32_t slot = __float_as_uint(sh[3 * e].w);
+        const float4 C = sh[3 * e + 2];
+        const float4 L = q.pathL[slot];
+        q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
+    };
+    traceWaveRange<W, true, false>(s, st, begin, end, refill, load, store, cnt, traced);
     const int slots[1] = {kStatOverflow};
-    const uint32_t vals[1] = {overflow};
+    const uint32_t vals[1] = {(uint32_t)st.overflow};
     blockFlush<1>(counters, slots, vals);
 }
 
 // One thread per distinct pixel of the generation; slots j, j+npad, ... hold
 // the same pixel, so accumulation needs no atomics.
-__global__ __launch_bounds__(kMaxBlock) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __restrict__ accum,
+EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __restrict__ accum,
                               uint32_t* __restrict__ counts, float4* __restrict__ aov,
                               unsigned long long* counters) {
     uint32_t stride = gridDim.x * blockDim.x;
@@ -546,7 +597,7 @@ hipError_t launchTrace(const LaunchCfg& c, const DevScene& s, const float* rays,
     size_t lds = sizeof(int) * kShortStack * c.block;
 #define EL_TRACE(WW, A, C)                                                                                   \
     hipLaunchKernelGGL((k_trace<WW, A, C>), dim3(grid), dim3(c.block), lds, c.stream, s, rays, tmax, n, tri, t, \
-                       uv, occ, c.gstack, c.maxThreads, counters)
+                       uv, occ, c.gstack, c.maxThreads, c.refill, counters)
     if (s.width == 4) {
         if (any)
             EL_TRACE(4, true, false);
@@ -581,8 +632,8 @@ hipError_t launchPlain(const LaunchCfg& c, const DevScene& s, const RenderArgs& 
 }
 
 hipError_t launchWfCamera(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, const WfState& q) {
-    uint32_t grid = gridFor(a.genCount, c.block, c.maxThreads / c.block);
-    hipLaunchKernelGGL(k_wf_camera, dim3(grid), dim3(c.block), 0, c.stream, s, a, q);
+    uint32_t grid = gridFor(a.genCount, c.shadeBlock, c.maxThreads / c.shadeBlock);
+    hipLaunchKernelGGL(k_wf_camera, dim3(grid), dim3(c.shadeBlock), 0, c.stream, s, a, q);
     return hipGetLastError();
 }
 
@@ -591,20 +642,29 @@ hipError_t launchWfExtend(const LaunchCfg& c, const DevScene& s, const WfState& 
                           unsigned long long* counters) {
     uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
-    if (s.width == 4)
-        hipLaunchKernelGGL(k_wf_extend<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetNext,
-                           countTraversal ? 1 : 0, counters);
-    else
-        hipLaunchKernelGGL(k_wf_extend<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, resetNext,
-                           countTraversal ? 1 : 0, counters);
+#define EL_EXT(WW, C)                                                                                       \
+    hipLaunchKernelGGL((k_wf_extend<WW, C>), dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, \
+                       resetNext, c.refill, counters)
+    if (s.width == 4) {
+        if (countTraversal)
+            EL_EXT(4, true);
+        else
+            EL_EXT(4, false);
+    } else {
+        if (countTraversal)
+            EL_EXT(8, true);
+        else
+            EL_EXT(8, false);
+    }
+#undef EL_EXT
     return hipGetLastError();
 }
 
 hipError_t launchWfShade(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, const WfState& q,
                          const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
                          uint32_t maxCount, unsigned long long* counters) {
-    uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
-    hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(c.block), 0, c.stream, s, a, q, rayIn, countIn, rayOut,
+    uint32_t grid = gridFor(maxCount, c.shadeBlock, c.maxThreads / c.shadeBlock);
+    hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(c.shadeBlock), 0, c.stream, s, a, q, rayIn, countIn, rayOut,
                        countOut, counters);
     return hipGetLastError();
 }
@@ -615,18 +675,18 @@ hipError_t launchWfShadow(const LaunchCfg& c, const DevScene& s, const WfState& 
     size_t lds = sizeof(int) * kShortStack * c.block;
     if (s.width == 4)
         hipLaunchKernelGGL(k_wf_shadow<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, shadow, countIn,
-                           counters);
+                           c.refill, counters);
     else
         hipLaunchKernelGGL(k_wf_shadow<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, shadow, countIn,
-                           counters);
+                           c.refill, counters);
     return hipGetLastError();
 }
 
 hipError_t launchWfFinalize(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, const WfState& q,
                             float4* accum, uint32_t* counts, float4* aov, unsigned long long* counters) {
     uint32_t lim = a.genCount < a.npad ? a.genCount : a.npad;
-    uint32_t grid = gridFor(lim, c.block, c.maxThreads / c.block);
-    hipLaunchKernelGGL(k_wf_finalize, dim3(grid), dim3(c.block), 0, c.stream, s, a, q, accum, counts, aov,
+    uint32_t grid = gridFor(lim, c.shadeBlock, c.maxThreads / c.shadeBlock);
+    hipLaunchKernelGGL(k_wf_finalize, dim3(grid), dim3(c.shadeBlock), 0, c.stream, s, a, q, accum, counts, aov,
                        counters);
     return hipGetLastError();
 }

@@ -16,6 +16,11 @@ constexpr int kXcds = 8;
 // Largest workgroup the kernels are compiled for (launch bounds).
 constexpr int kMaxBlock = 256;
 
+// Tuned defaults (heightfield 1M tris at 1080p on MI355X): traversal block
+// size and the idle-lane count that triggers an in-wave ray refill.
+constexpr uint32_t kDefaultBlock = 64;
+constexpr uint32_t kDefaultRefill = 32;
+
 // Tally slots (per XCD copy): the host sums the kXcds copies.
 constexpr int kStatSlots = 8;
 constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5;
@@ -52,8 +57,10 @@ struct WfState {
 
 struct LaunchCfg {
     hipStream_t stream;
-    uint32_t block;
+    uint32_t block;         // traversal kernels (trace, extend, shadow) and the plain megakernel
+    uint32_t shadeBlock;    // camera, shade, finalize
     uint32_t maxThreads;    // grid * block upper bound (size of the spill area)
+    uint32_t refill;        // persistent traversal: idle lanes that trigger a refill
     int* gstack;
 };
 

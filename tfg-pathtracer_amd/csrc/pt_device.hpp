@@ -546,17 +546,120 @@ struct TraceCounters {
     uint32_t nodes, tris;
 };
 
+// A ray prepared for slab tests: origin, direction and safe reciprocal.
+struct TRay {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+};
+
+__device__ __forceinline__ TRay makeTRay(float ox, float oy, float oz, float dx, float dy, float dz) {
+    const float tiny = 1e-30f;
+    TRay r;
+    r.ox = ox;
+    r.oy = oy;
+    r.oz = oz;
+    r.dx = dx;
+    r.dy = dy;
+    r.dz = dz;
+    r.ix = 1.0f / (fabsf(dx) > tiny ? dx : copysignf(tiny, dx));
+    r.iy = 1.0f / (fabsf(dy) > tiny ? dy : copysignf(tiny, dy));
+    r.iz = 1.0f / (fabsf(dz) > tiny ? dz : copysignf(tiny, dz));
+    return r;
+}
+
+// Visits inner node `cur`: slab-tests its W children against [0, tcull],
+// pushes the hit children far-to-near and returns the nearest (or the next
+// stack entry if none is hit; -1 when the stack is empty).  Child encoding:
+// >= 0 inner node, <= -2 leaf, -1 empty slot (its box is inverted, so it is
+// never hit).
+template <int W>
+__device__ __forceinline__ int visitNode(const DNode<W>* __restrict__ nodes, int cur, const TRay& ray, float tcull,
+                                         TravStack& st) {
+    const DNode<W>& n = nodes[cur];
+    const bool nx = ray.ix < 0, ny = ray.iy < 0, nz = ray.iz < 0;
+    uint32_t keys[W];
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        float bx0 = nx ? n.hix[i] : n.lox[i], bx1 = nx ? n.lox[i] : n.hix[i];
+        float by0 = ny ? n.hiy[i] : n.loy[i], by1 = ny ? n.loy[i] : n.hiy[i];
+        float bz0 = nz ? n.hiz[i] : n.loz[i], bz1 = nz ? n.loz[i] : n.hiz[i];
+        float tn = fmaxf(fmaxf((bx0 - ray.ox) * ray.ix, (by0 - ray.oy) * ray.iy), fmaxf((bz0 - ray.oz) * ray.iz, 0.0f));
+        float tf = fminf(fminf((bx1 - ray.ox) * ray.ix, (by1 - ray.oy) * ray.iy), fminf((bz1 - ray.oz) * ray.iz, tcull));
+        keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+    }
+    // ascending sort (tn >= 0, so float bits order as unsigned)
+#define EL_CSWAP(a, b)                        \
+    {                                         \
+        uint32_t lo_ = min(keys[a], keys[b]); \
+        uint32_t hi_ = max(keys[a], keys[b]); \
+        keys[a] = lo_;                        \
+        keys[b] = hi_;                        \
+    }
+    if (W == 4) {
+        EL_CSWAP(0, 1) EL_CSWAP(2, 3) EL_CSWAP(0, 2) EL_CSWAP(1, 3) EL_CSWAP(1, 2)
+    } else {
+#pragma unroll
+        for (int i = 1; i < W; i++)
+#pragma unroll
+            for (int j = i; j > 0; j--) EL_CSWAP(j - 1, j)
+    }
+#undef EL_CSWAP
+    int nh = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) nh += keys[i] != 0xFFFFFFFFu;
+    if (nh == 0) return st.pop();
+#pragma unroll
+    for (int i = W - 1; i >= 1; i--)
+        if (i < nh) st.push(n.child[keys[i] & 7u]);
+    return n.child[keys[0] & 7u];
+}
+
+// Tests the triangles of leaf `code` (<= -2) and updates the closest hit:
+// smaller t wins, equal t goes to the lower original index.  For ANY the
+// first hit below r.t is taken and true is returned.
+template <bool ANY, bool COUNT>
+__device__ __forceinline__ bool testLeaf(const float4* __restrict__ tris, int code, const TRay& ray, TraceResult& r,
+                                         TraceCounters& cnt) {
+    int k = -code - 2;
+    int first = k >> 4, count = (k & 15) + 1;
+    for (int i = 0; i < count; i++) {
+        const float4* p = tris + (uint64_t)(first + i) * 3;
+        float4 A = p[0], B = p[1], C = p[2];
+        if (COUNT) cnt.tris++;
+        float t, u, v;
+        if (mtHit(A, B, C, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, t, u, v)) {
+            int id = __float_as_int(C.y);
+            if (ANY) {
+                if (t < r.t) {
+                    r.tri = id;
+                    r.t = t;
+                    r.u = u;
+                    r.v = v;
+                    return true;
+                }
+            } else if (t < r.t || (t == r.t && (unsigned)id < (unsigned)r.tri)) {
+                r.tri = id;
+                r.t = t;
+                r.u = u;
+                r.v = v;
+            }
+        }
+    }
+    return false;
+}
+
+// Culling bound for child boxes: closest-hit keeps boxes slightly beyond the
+// current hit so an equal-t hit with a lower index is still found.
+template <bool ANY>
+__device__ __forceinline__ float cullT(float t) {
+    return ANY ? t : t * (1.0f + 1.0f / 1048576.0f);
+}
+
+// One ray, one thread: depth-first traversal with nearest-child-first order.
 template <int W, bool ANY, bool COUNT>
 __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, float oy, float oz, float dx, float dy,
                                                 float dz, float tmax, TravStack& st, TraceCounters& cnt) {
-    using Node = DNode<W>;
-    const Node* __restrict__ nodes = (const Node*)s.nodes;
-    const float4* __restrict__ tris = s.tris;
-    const float tiny = 1e-30f;
-    float ix = 1.0f / (fabsf(dx) > tiny ? dx : copysignf(tiny, dx));
-    float iy = 1.0f / (fabsf(dy) > tiny ? dy : copysignf(tiny, dy));
-    float iz = 1.0f / (fabsf(dz) > tiny ? dz : copysignf(tiny, dz));
-    const bool nx = ix < 0, ny = iy < 0, nz = iz < 0;
+    const DNode<W>* __restrict__ nodes = (const DNode<W>*)s.nodes;
+    const TRay ray = makeTRay(ox, oy, oz, dx, dy, dz);
     TraceResult r;
     r.tri = -1;
     r.t = tmax;
@@ -566,79 +669,115 @@ __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, flo
     int cur = 0;
     while (cur != -1) {
         if (cur >= 0) {
-            const Node& n = nodes[cur];
             if (COUNT) cnt.nodes++;
-            float tcull = ANY ? r.t : r.t * (1.0f + 1.0f / 1048576.0f);
-            uint32_t keys[W];
-#pragma unroll
-            for (int i = 0; i < W; i++) {
-                float bx0 = nx ? n.hix[i] : n.lox[i], bx1 = nx ? n.lox[i] : n.hix[i];
-                float by0 = ny ? n.hiy[i] : n.loy[i], by1 = ny ? n.loy[i] : n.hiy[i];
-                float bz0 = nz ? n.hiz[i] : n.loz[i], bz1 = nz ? n.loz[i] : n.hiz[i];
-                float tn = fmaxf(fmaxf((bx0 - ox) * ix, (by0 - oy) * iy), fmaxf((bz0 - oz) * iz, 0.0f));
-                float tf = fminf(fminf((bx1 - ox) * ix, (by1 - oy) * iy), fminf((bz1 - oz) * iz, tcull));
-                keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
-            }
-            // sort ascending (small networks)
-            if (W == 4) {
-#define EL_CSWAP(a, b)                         \
-    {                                          \
-        uint32_t lo_ = min(keys[a], keys[b]);  \
-        uint32_t hi_ = max(keys[a], keys[b]);  \
-        keys[a] = lo_;                         \
-        keys[b] = hi_;                         \
-    }
-                EL_CSWAP(0, 1) EL_CSWAP(2, 3) EL_CSWAP(0, 2) EL_CSWAP(1, 3) EL_CSWAP(1, 2)
-            } else {
-#pragma unroll
-                for (int i = 1; i < W; i++)
-#pragma unroll
-                    for (int j = i; j > 0; j--) EL_CSWAP(j - 1, j)
-#undef EL_CSWAP
-            }
-            int nh = 0;
-#pragma unroll
-            for (int i = 0; i < W; i++) nh += keys[i] != 0xFFFFFFFFu;
-            if (nh == 0) {
-                cur = st.pop();
-            } else {
-#pragma unroll
-                for (int i = W - 1; i >= 1; i--)
-                    if (i < nh) st.push(n.child[keys[i] & 7u]);
-                cur = n.child[keys[0] & 7u];
-            }
+            cur = visitNode<W>(nodes, cur, ray, cullT<ANY>(r.t), st);
         } else {
-            int k = -cur - 2;
-            int first = k >> 4, count = (k & 15) + 1;
-            bool done = false;
-            for (int i = 0; i < count; i++) {
-                const float4* p = tris + (uint64_t)(first + i) * 3;
-                float4 A = p[0], B = p[1], C = p[2];
-                if (COUNT) cnt.tris++;
-                float t, u, v;
-                if (mtHit(A, B, C, ox, oy, oz, dx, dy, dz, t, u, v)) {
-                    int id = __float_as_int(C.y);
-                    if (ANY) {
-                        if (t < r.t) {
-                            r.tri = id;
-                            r.t = t;
-                            r.u = u;
-                            r.v = v;
-                            done = true;
-                            break;
-                        }
-                    } else if (t < r.t || (t == r.t && (unsigned)id < (unsigned)r.tri)) {
-                        r.tri = id;
-                        r.t = t;
-                        r.u = u;
-                        r.v = v;
-                    }
-                }
-            }
+            bool done = testLeaf<ANY, COUNT>(s.tris, cur, ray, r, cnt);
             cur = done ? -1 : st.pop();
         }
     }
     return r;
+}
+
+// Persistent traversal of the items [begin, end) owned by one wave, with
+// in-wave refill: a lane whose ray is finished takes the wave's next item, so
+// the wave keeps its lanes busy instead of waiting for its slowest ray (the
+// refill happens once at least `refill` lanes are idle, or all are).  Inner
+// nodes and leaves are separated (Aila & Laine, "Understanding the Efficiency
+// of Ray Traversal on GPUs", HPG 2009, speculative while-while): a lane that
+// reaches a leaf postpones it and keeps descending until every live lane of
+// the wave holds a leaf, then the wave tests leaves together.  The closest
+// hit is order-independent (t, then lower index), so results are identical
+// to traceRay.
+//
+//   load(i, ox, oy, oz, dx, dy, dz, tmax) -> false if item i has no ray
+//   store(i, result, traced)              once per item
+//
+// Every lane of the wave must call with the same begin/end.
+template <int W, bool ANY, bool COUNT, class Load, class Store>
+__device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st, uint32_t begin, uint32_t end,
+                                               uint32_t refill, Load&& load, Store&& store, TraceCounters& cnt,
+                                               uint32_t& traced) {
+    const DNode<W>* __restrict__ nodes = (const DNode<W>*)s.nodes;
+    const uint32_t lane = __lane_id();
+    const unsigned long long ltMask = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t cursor = begin;   // wave-uniform
+    bool live = false;
+    uint32_t item = 0;
+    TRay ray{};
+    TraceResult r{};
+    int cur = -1, leaf = -1;
+    while (true) {
+        // ---- refill idle lanes from the wave's range
+        const unsigned long long idle = __ballot(!live);
+        const uint32_t nIdle = (uint32_t)__popcll(idle);
+        if (cursor < end && (nIdle >= refill || nIdle == 64)) {
+            if (!live) {
+                uint32_t i = cursor + (uint32_t)__popcll(idle & ltMask);
+                if (i < end) {
+                    float ox, oy, oz, dx, dy, dz, tmax;
+                    if (load(i, ox, oy, oz, dx, dy, dz, tmax)) {
+                        ray = makeTRay(ox, oy, oz, dx, dy, dz);
+                        r.tri = -1;
+                        r.t = tmax;
+                        r.u = r.v = 0.f;
+                        st.sp = 0;
+                        st.lo = 0;
+                        cur = 0;
+                        leaf = -1;
+                        item = i;
+                        live = true;
+                    } else {
+                        TraceResult dead;
+                        dead.tri = -2;
+                        dead.t = dead.u = dead.v = 0.f;
+                        store(i, dead, false);
+                    }
+                }
+            }
+            cursor += nIdle;
+        } else if (nIdle == 64) {
+            break;   // range exhausted and every lane finished
+        }
+        // ---- inner nodes: descend until every live lane holds a leaf
+        while (true) {
+            const bool act = live && cur >= 0;
+            if (__all(!act || leaf != -1)) break;   // also when no lane can descend
+            if (act) {
+                if (COUNT) cnt.nodes++;
+                cur = visitNode<W>(nodes, cur, ray, cullT<ANY>(r.t), st);
+                if (cur <= -2 && leaf == -1) {
+                    leaf = cur;
+                    cur = st.pop();
+                }
+            }
+        }
+        // ---- leaves: the postponed one, then any leaves that follow on the stack
+        while (true) {
+            const bool has = live && (leaf != -1 || cur <= -2);
+            if (!__any(has)) break;
+            if (has) {
+                int code;
+                if (leaf != -1) {
+                    code = leaf;
+                    leaf = -1;
+                } else {
+                    code = cur;
+                    cur = st.pop();
+                }
+                if (testLeaf<ANY, COUNT>(s.tris, code, ray, r, cnt)) {
+                    cur = -1;   // any-hit: done
+                    leaf = -1;
+                }
+            }
+        }
+        // ---- finished rays
+        if (live && cur == -1 && leaf == -1) {
+            store(item, r, true);
+            traced++;
+            live = false;
+        }
+    }
 }
 
 }  // namespace dev

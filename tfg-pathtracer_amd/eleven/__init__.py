@@ -77,7 +77,7 @@ class el_scene_view(C.Structure):
 class el_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("mode", C.c_int32),
                 ("block", C.c_uint32), ("queue_cap", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("flags", C.c_uint32)]
+                ("flags", C.c_uint32), ("refill", C.c_uint32)]
 
 
 class el_stats(C.Structure):
@@ -124,12 +124,15 @@ _lib = None
 
 
 def lib():
-    """Load libeleven.so (build it with `make -C tfg-pathtracer_amd`)."""
+    """Load libeleven.so (build it with `make -C tfg-pathtracer_amd`).  The
+    environment variable ELEVEN_LIBRARY names another build of the same
+    library (e.g. one compiled with different kernel tuning macros)."""
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
-            raise ElevenError(EL_ERR_STATE, f"{LIB_PATH} is not built; run __graft_entry__.build()")
-        L = C.CDLL(str(LIB_PATH))
+        path = Path(os.environ.get("ELEVEN_LIBRARY") or LIB_PATH)
+        if not path.exists():
+            raise ElevenError(EL_ERR_STATE, f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(path))
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)
             f.restype = res
@@ -328,8 +331,9 @@ class Context:
         self._check(lib().el_upload_scene(self._h, C.byref(scene.view)))
         self.scene = scene
 
-    def render(self, spp, depth=5, seed=0, mode=MODE_WAVEFRONT, block=0, queue_cap=0, bvh_width=0, flags=0):
-        p = el_render_params(spp, depth, seed, mode, block, queue_cap, bvh_width, flags)
+    def render(self, spp, depth=5, seed=0, mode=MODE_WAVEFRONT, block=0, queue_cap=0, bvh_width=0, flags=0,
+               refill=0):
+        p = el_render_params(spp, depth, seed, mode, block, queue_cap, bvh_width, flags, refill)
         self._check(lib().el_render(self._h, C.byref(p)))
         return self.stats()
 
