@@ -29,8 +29,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "tfg-pathtracer_amd"))
 
 HBM_PEAK_GBS = 8000.0
-S_NODE = {4: 128, 8: 256}   # bytes per wide node as stored
-S_TRI = 48                  # bytes per triangle record as stored
 
 
 def log(msg):
@@ -167,7 +165,8 @@ def main():
         ctx.clear()
         sc = ctx.render(args.spp, seed=0, flags=eleven.FLAG_COUNT_TRAVERSAL, **kw)
         per_frame_launches = sc.extend_launches
-        algo_bytes = sc.node_fetches * S_NODE[width] + sc.tri_tests * S_TRI
+        s_node, s_tri = sc.node_stride, sc.tri_stride   # bytes per node / triangle record as stored
+        algo_bytes = sc.node_fetches * s_node + sc.tri_tests * s_tri
         bytes_per_launch = algo_bytes / max(1, per_frame_launches)
         avg_ms = ext_ms / ext_launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
@@ -178,7 +177,7 @@ def main():
                            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_frame_launches,
                            "algo_bytes_per_launch": int(bytes_per_launch),
                            "node_fetches_per_step": sc.node_fetches, "tri_tests_per_step": sc.tri_tests,
-                           "bytes_per_node": S_NODE[width], "bytes_per_tri": S_TRI}
+                           "bytes_per_node": s_node, "bytes_per_tri": s_tri}
         out["config"]["extend_ms_per_step"] = round(ext_ms / args.steps, 2)
         out["config"]["shade_ms_per_step"] = round(st.shade_ms, 2)
         out["config"]["shadow_ms_per_step"] = round(st.shadow_ms, 2)

@@ -137,6 +137,8 @@ typedef struct {
     uint64_t tri_bytes;        /* size of the device triangle stream         */
     uint32_t bvh_width;
     uint32_t stack_bound;      /* worst-case traversal stack depth           */
+    uint32_t node_stride;      /* bytes per BVH node as stored (64 or 96)    */
+    uint32_t tri_stride;       /* bytes per triangle record as stored (48)   */
 } el_stats;
 
 /* ---- host-side scene loading (SceneLoader.hpp:7 loadScene) ------------ */

@@ -284,13 +284,74 @@ void packWide(const Bvh2& b, const float pad, WideBvh& out) {
         wn[it.wide] = node;
     }
     out.nodeCount = wn.size();
-    out.nodes.resize(wn.size() * sizeof(WideNode<W>));
-    std::memcpy(out.nodes.data(), wn.data(), out.nodes.size());
+    out.nodeStride = sizeof(QNode<W>);
+    std::vector<QNode<W>> qn(wn.size());
+    for (size_t i = 0; i < wn.size(); i++)
+        if (!quantizeNode<W>(wn[i], qn[i])) throw std::runtime_error("non-finite bounding box (vertex data)");
+    out.nodes.resize(qn.size() * sizeof(QNode<W>));
+    std::memcpy(out.nodes.data(), qn.data(), out.nodes.size());
     out.stackBound = std::max(1, maxStack);
     out.depth = maxDepth;
 }
 
 }  // namespace
+
+template <int W>
+bool quantizeNode(const WideNode<W>& in, QNode<W>& out) {
+    std::memset(&out, 0, sizeof out);
+    const float* lo[3] = {in.lox, in.loy, in.loz};
+    const float* hi[3] = {in.hix, in.hiy, in.hiz};
+    uint8_t qlo[3][W], qhi[3][W];
+    for (int a = 0; a < 3; a++) {
+        double L = std::numeric_limits<double>::infinity(), H = -L;
+        for (int i = 0; i < W; i++) {
+            if (in.child[i] == -1) continue;
+            if (!std::isfinite(lo[a][i]) || !std::isfinite(hi[a][i])) return false;
+            L = std::min(L, (double)lo[a][i]);
+            H = std::max(H, (double)hi[a][i]);
+        }
+        int e = 0;
+        double p = 0, step = 1;
+        if (L <= H) {
+            // smallest step with the extent inside 253 steps, then coarser
+            // until the grid origin's multiple stays below 2^24 - 256
+            double ext = std::max(H - L, 1e-30);
+            e = std::max(-126, (int)std::ceil(std::log2(ext / 253.0)));
+            for (;; e++) {
+                if (e > 127) return false;
+                step = std::ldexp(1.0, e);
+                double k = std::floor(L / step);
+                if (std::fabs(k) + 256.0 >= 16777216.0) continue;
+                p = k * step;
+                if (std::ceil((H - p) / step) > 255.0) continue;
+                break;
+            }
+        }
+        out.p[a] = (float)p;   // exact: |p / step| < 2^24
+        out.exps |= (uint32_t)(e + 127) << (8 * a);
+        for (int i = 0; i < W; i++) {
+            if (in.child[i] == -1 || !(L <= H)) {
+                qlo[a][i] = 255;   // inverted: never intersected
+                qhi[a][i] = 0;
+                continue;
+            }
+            double ql = std::floor(((double)lo[a][i] - p) / step);
+            double qh = std::ceil(((double)hi[a][i] - p) / step);
+            qlo[a][i] = (uint8_t)std::min(255.0, std::max(0.0, ql));
+            qhi[a][i] = (uint8_t)std::min(255.0, std::max(0.0, qh));
+        }
+    }
+    for (int a = 0; a < 3; a++)
+        for (int i = 0; i < W; i++) {
+            out.q[a * 2 + 0][i / 4] |= (uint32_t)qlo[a][i] << (8 * (i % 4));
+            out.q[a * 2 + 1][i / 4] |= (uint32_t)qhi[a][i] << (8 * (i % 4));
+        }
+    for (int i = 0; i < W; i++) out.child[i] = in.child[i];
+    return true;
+}
+
+template bool quantizeNode<4>(const WideNode<4>&, QNode<4>&);
+template bool quantizeNode<8>(const WideNode<8>&, QNode<8>&);
 
 WideBvh collapseWide(const Bvh2& b, const float* v, int n, int width) {
     if (width != 4 && width != 8) throw std::runtime_error("BVH width must be 4 or 8");

@@ -27,6 +27,36 @@ struct alignas(16) WideNode {
 static_assert(sizeof(WideNode<4>) == 128, "BVH4 node must be 128 B");
 static_assert(sizeof(WideNode<8>) == 256, "BVH8 node must be 256 B");
 
+// Quantized wide node, the layout the device traverses.  Per axis a, child
+// box planes lie on a grid: plane = p[a] + q * 2^(e[a]-127) with q an 8-bit
+// integer.  p[a] is a multiple of the grid step with |p/step| + 255 < 2^24,
+// so every plane p + q*step is exactly representable in FP32 and the device
+// decodes it exactly (one fma).  Quantization rounds outwards (floor for lo,
+// ceil for hi), so each quantized box contains the (padded) FP32 box.
+//   p[3], exps (e_x | e_y << 8 | e_z << 16)          16 B
+//   q[axis*2 + {0 lo, 1 hi}][W/4] bytes, child i in byte i%4 of dword i/4
+//   child[W] (same encoding as WideNode)
+// W = 4: 56 B used, padded to 64 (4 loads); W = 8: 96 B (6 loads).
+template <int W>
+struct QNode;
+template <>
+struct alignas(16) QNode<4> {
+    float p[3];
+    uint32_t exps;
+    uint32_t q[6][1];
+    int32_t child[4];
+    int32_t pad[2];
+};
+template <>
+struct alignas(16) QNode<8> {
+    float p[3];
+    uint32_t exps;
+    uint32_t q[6][2];
+    int32_t child[8];
+};
+static_assert(sizeof(QNode<4>) == 64, "quantized BVH4 node must be 64 B");
+static_assert(sizeof(QNode<8>) == 96, "quantized BVH8 node must be 96 B");
+
 constexpr int kMaxLeafTris = 8;        // <= 16 fits the leaf encoding
 constexpr int kLeafFirstLimit = 1 << 27;
 
@@ -46,7 +76,8 @@ struct Bvh2 {
 
 struct WideBvh {
     int width = 4;
-    std::vector<unsigned char> nodes;   // nodeCount * sizeof(WideNode<W>)
+    std::vector<unsigned char> nodes;   // nodeCount * sizeof(QNode<W>)
+    size_t nodeStride = 0;              // bytes per node as stored
     size_t nodeCount = 0;
     std::vector<float> triStream;       // 12 floats per leaf-order triangle
     std::vector<int> order;             // leaf order -> original index
@@ -61,5 +92,10 @@ Bvh2 buildBvh2(const float* vertices, int n, int threads = 0);
 // outwards by `pad` (absolute, in scene units) so the slab test is
 // conservative with respect to the Moller-Trumbore acceptance region.
 WideBvh collapseWide(const Bvh2& b, const float* vertices, int n, int width);
+
+// Quantizes one FP32 wide node (boxes already padded); returns false if a
+// child box is not finite.
+template <int W>
+bool quantizeNode(const WideNode<W>& in, QNode<W>& out);
 
 }  // namespace eleven

@@ -199,6 +199,8 @@ static int uploadBvh(el_ctx* c, int width) {
     c->stats.stack_bound = (uint32_t)c->wide.stackBound;
     c->stats.node_bytes = c->wide.nodes.size();
     c->stats.tri_bytes = c->wide.triStream.size() * sizeof(float);
+    c->stats.node_stride = (uint32_t)c->wide.nodeStride;
+    c->stats.tri_stride = 12 * sizeof(float);
     EL_HIP(c, hipStreamSynchronize(c->stream));
     return EL_OK;
 }

@@ -19,7 +19,7 @@ constexpr int kMaxBlock = 256;
 // Tuned defaults (heightfield 1M tris at 1080p on MI355X): traversal block
 // size and the idle-lane count that triggers an in-wave ray refill.
 constexpr uint32_t kDefaultBlock = 64;
-constexpr uint32_t kDefaultRefill = 32;
+constexpr uint32_t kDefaultRefill = 16;
 
 // Tally slots (per XCD copy): the host sums the kXcds copies.
 constexpr int kStatSlots = 8;

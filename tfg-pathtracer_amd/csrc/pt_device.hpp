@@ -529,12 +529,12 @@ struct TravStack {
     }
 };
 
-// Same layout as eleven::WideNode<W> (bvh.hpp).
+// Quantized wide node as stored (eleven::QNode<W>, bvh.hpp): 16-byte words
+// per node, W = 4: 4 (header, q0-3, q4-5 + child0-1, child2-3 + pad);
+// W = 8: 6 (header, q0-11 in three words, child0-7 in two).
 template <int W>
-struct alignas(16) DNode {
-    float lox[W], loy[W], loz[W], hix[W], hiy[W], hiz[W];
-    int32_t child[W];
-    int32_t pad[W];
+struct QNodeWords {
+    static constexpr int kWords = W == 4 ? 4 : 6;
 };
 
 struct TraceResult {
@@ -571,19 +571,60 @@ __device__ __forceinline__ TRay makeTRay(float ox, float oy, float oz, float dx,
 // stack entry if none is hit; -1 when the stack is empty).  Child encoding:
 // >= 0 inner node, <= -2 leaf, -1 empty slot (its box is inverted, so it is
 // never hit).
+//
+// Planes decode exactly (p + q * step is an FP32 value by construction), so
+// the computed slab distances (b - o) * (1/d) carry only relative rounding
+// error (< 2^-21 with the reciprocal's); widening tfar by 2^-20 of its
+// magnitude makes the test conservative: no box the ray touches is culled.
 template <int W>
-__device__ __forceinline__ int visitNode(const DNode<W>* __restrict__ nodes, int cur, const TRay& ray, float tcull,
+__device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cur, const TRay& ray, float tcull,
                                          TravStack& st) {
-    const DNode<W>& n = nodes[cur];
+    constexpr int D = W / 4;   // dwords of bytes per (axis, side)
+    const uint4* np = nodes + (uint64_t)cur * QNodeWords<W>::kWords;
+    const uint4 h = np[0];
+    uint32_t q[6 * D];
+    int32_t child[W];
+    if constexpr (W == 4) {
+        const uint4 a = np[1], b = np[2], c = np[3];
+        q[0] = a.x, q[1] = a.y, q[2] = a.z, q[3] = a.w, q[4] = b.x, q[5] = b.y;
+        child[0] = (int32_t)b.z, child[1] = (int32_t)b.w, child[2] = (int32_t)c.x, child[3] = (int32_t)c.y;
+    } else {
+        const uint4 a = np[1], b = np[2], c = np[3], d = np[4], e = np[5];
+        q[0] = a.x, q[1] = a.y, q[2] = a.z, q[3] = a.w, q[4] = b.x, q[5] = b.y;
+        q[6] = b.z, q[7] = b.w, q[8] = c.x, q[9] = c.y, q[10] = c.z, q[11] = c.w;
+        child[0] = (int32_t)d.x, child[1] = (int32_t)d.y, child[2] = (int32_t)d.z, child[3] = (int32_t)d.w;
+        child[4] = (int32_t)e.x, child[5] = (int32_t)e.y, child[6] = (int32_t)e.z, child[7] = (int32_t)e.w;
+    }
+    const float px = __uint_as_float(h.x), py = __uint_as_float(h.y), pz = __uint_as_float(h.z);
+    const float sx = __uint_as_float((h.w & 0xffu) << 23);
+    const float sy = __uint_as_float(((h.w >> 8) & 0xffu) << 23);
+    const float sz = __uint_as_float(((h.w >> 16) & 0xffu) << 23);
+    // near / far plane bytes per axis by the direction's sign (q index: axis*2*D + side*D + dword)
     const bool nx = ray.ix < 0, ny = ray.iy < 0, nz = ray.iz < 0;
+    uint32_t nearX[D], farX[D], nearY[D], farY[D], nearZ[D], farZ[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        nearX[d] = nx ? q[1 * D + d] : q[0 * D + d];
+        farX[d] = nx ? q[0 * D + d] : q[1 * D + d];
+        nearY[d] = ny ? q[3 * D + d] : q[2 * D + d];
+        farY[d] = ny ? q[2 * D + d] : q[3 * D + d];
+        nearZ[d] = nz ? q[5 * D + d] : q[4 * D + d];
+        farZ[d] = nz ? q[4 * D + d] : q[5 * D + d];
+    }
     uint32_t keys[W];
 #pragma unroll
     for (int i = 0; i < W; i++) {
-        float bx0 = nx ? n.hix[i] : n.lox[i], bx1 = nx ? n.lox[i] : n.hix[i];
-        float by0 = ny ? n.hiy[i] : n.loy[i], by1 = ny ? n.loy[i] : n.hiy[i];
-        float bz0 = nz ? n.hiz[i] : n.loz[i], bz1 = nz ? n.loz[i] : n.hiz[i];
-        float tn = fmaxf(fmaxf((bx0 - ray.ox) * ray.ix, (by0 - ray.oy) * ray.iy), fmaxf((bz0 - ray.oz) * ray.iz, 0.0f));
+        const int d = i >> 2, sh = 8 * (i & 3);
+        const float bx0 = fmaf((float)((nearX[d] >> sh) & 0xffu), sx, px);
+        const float bx1 = fmaf((float)((farX[d] >> sh) & 0xffu), sx, px);
+        const float by0 = fmaf((float)((nearY[d] >> sh) & 0xffu), sy, py);
+        const float by1 = fmaf((float)((farY[d] >> sh) & 0xffu), sy, py);
+        const float bz0 = fmaf((float)((nearZ[d] >> sh) & 0xffu), sz, pz);
+        const float bz1 = fmaf((float)((farZ[d] >> sh) & 0xffu), sz, pz);
+        const float tn = fmaxf(fmaxf((bx0 - ray.ox) * ray.ix, (by0 - ray.oy) * ray.iy),
+                               fmaxf((bz0 - ray.oz) * ray.iz, 0.0f));
         float tf = fminf(fminf((bx1 - ray.ox) * ray.ix, (by1 - ray.oy) * ray.iy), fminf((bz1 - ray.oz) * ray.iz, tcull));
+        tf = fmaf(fabsf(tf), 1.0f / 1048576.0f, tf);
         keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
     }
     // ascending sort (tn >= 0, so float bits order as unsigned)
@@ -609,8 +650,8 @@ __device__ __forceinline__ int visitNode(const DNode<W>* __restrict__ nodes, int
     if (nh == 0) return st.pop();
 #pragma unroll
     for (int i = W - 1; i >= 1; i--)
-        if (i < nh) st.push(n.child[keys[i] & 7u]);
-    return n.child[keys[0] & 7u];
+        if (i < nh) st.push(child[keys[i] & 7u]);
+    return child[keys[0] & 7u];
 }
 
 // Tests the triangles of leaf `code` (<= -2) and updates the closest hit:
@@ -658,7 +699,7 @@ __device__ __forceinline__ float cullT(float t) {
 template <int W, bool ANY, bool COUNT>
 __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, float oy, float oz, float dx, float dy,
                                                 float dz, float tmax, TravStack& st, TraceCounters& cnt) {
-    const DNode<W>* __restrict__ nodes = (const DNode<W>*)s.nodes;
+    const uint4* __restrict__ nodes = (const uint4*)s.nodes;
     const TRay ray = makeTRay(ox, oy, oz, dx, dy, dz);
     TraceResult r;
     r.tri = -1;
@@ -698,7 +739,7 @@ template <int W, bool ANY, bool COUNT, class Load, class Store>
 __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st, uint32_t begin, uint32_t end,
                                                uint32_t refill, Load&& load, Store&& store, TraceCounters& cnt,
                                                uint32_t& traced) {
-    const DNode<W>* __restrict__ nodes = (const DNode<W>*)s.nodes;
+    const uint4* __restrict__ nodes = (const uint4*)s.nodes;
     const uint32_t lane = __lane_id();
     const unsigned long long ltMask = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint32_t cursor = begin;   // wave-uniform

@@ -86,7 +86,7 @@ class el_stats(C.Structure):
                 ("shade_ms", C.c_double), ("extend_launches", C.c_uint64), ("extend_rays", C.c_uint64),
                 ("node_fetches", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64), ("bvh_width", C.c_uint32),
-                ("stack_bound", C.c_uint32)]
+                ("stack_bound", C.c_uint32), ("node_stride", C.c_uint32), ("tri_stride", C.c_uint32)]
 
 
 # Every symbol declared in include/eleven.h, with its ctypes signature.
