@@ -42,6 +42,20 @@ def ensure_built():
         __graft_entry__.build()
 
 
+def measured_traffic(kernel="k_wf_extend"):
+    """HBM bytes per launch of `kernel` on this workload at the default
+    settings, from the committed FETCH_SIZE / WRITE_SIZE counter summary
+    (rocprofv3 --pmc, scaled by a 1 GiB copy).  None if absent."""
+    f = ROOT / "profiles" / "hbm_traffic_heightfield_1080p_64spp.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+            return int(parts[2])
+    return None
+
+
 def cpu_baseline(scene, budget_s=15.0, threads=None):
     """Oracle render (FP32, reference estimator) of a centred crop of the frame,
     its sample count chosen from a short probe so the timed part takes about
@@ -171,8 +185,11 @@ def main():
         avg_ms = ext_ms / ext_launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         ext_rays = sc.rays - 0  # informational
+        defaults = (args.block, args.queue, args.refill, args.spp, args.depth, args.res) == (0, 0, 0, 64, 8, "1920x1080")
+        traffic = measured_traffic() if defaults and width == 4 else None
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "traffic_source": "profiles/hbm_traffic_heightfield_1080p_64spp.md" if traffic else None,
                            "kernel": "k_wf_extend (closest-hit traversal)",
                            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_frame_launches,
                            "algo_bytes_per_launch": int(bytes_per_launch),
