@@ -139,7 +139,18 @@ typedef struct {
     uint32_t stack_bound;      /* worst-case traversal stack depth           */
     uint32_t node_stride;      /* bytes per BVH node as stored (64 or 96)    */
     uint32_t tri_stride;       /* bytes per triangle record as stored (48)   */
+    double   adjoint_ms;       /* device time of the last el_adjoint         */
+    uint64_t adjoint_rays;     /* BVH queries of the last el_adjoint (primal
+                                  walk + replay)                             */
 } el_stats;
+
+/* Adjoint of the accumulated image (no counterpart in the reference, which
+ * renders forward only). */
+typedef struct {
+    int32_t  grad_texture; /* texture whose texels receive gradients, -1: none */
+    uint32_t block;        /* threads per block, 0 -> default               */
+    uint32_t flags;        /* reserved, 0                                   */
+} el_adjoint_params;
 
 /* ---- host-side scene loading (SceneLoader.hpp:7 loadScene) ------------ */
 typedef struct el_scene el_scene;
@@ -188,6 +199,22 @@ int  el_get_stats(el_ctx* ctx, el_stats* out);
 int  el_set_materials(el_ctx* ctx, const el_material* materials, uint32_t count);
 /* Replace the texels of texture `tex_id` (same size as uploaded). */
 int  el_set_texture(el_ctx* ctx, int32_t tex_id, const float* rgb);
+
+/* Path-replay adjoint (Vicini et al. 2021, detached sampling) of the image
+ * currently accumulated in the context (every el_render since el_clear,
+ * which must share one seed and depth <= 16): gradients of
+ *     sum over pixels and channels of d_image * I
+ * with respect to each material's (albedo r, g, b, roughness, metallic,
+ * specular) - the values stored in el_material, before the pow 2.2 of
+ * generateHitData (kernel.cu:103-104) - written to d_materials[M][6], and to
+ * the texels of texture params->grad_texture, written to d_texture[h][w][3]
+ * (may be NULL if grad_texture < 0).  d_image is [H][W][3] in the layout of
+ * el_read_passes.  Each path is replayed from its RNG counters, nothing of a
+ * path is stored; a material whose albedo comes from a texture gets no
+ * albedo gradient (its texels do, if selected).  With `device_ptrs` != 0 the
+ * three arrays are device pointers. */
+int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,
+                float* d_materials, float* d_texture, int device_ptrs);
 
 /* Closest hit for n rays: tri = original triangle index or -1, t = hit
  * distance along the (caller-normalised) direction, uv = barycentrics

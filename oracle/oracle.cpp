@@ -494,6 +494,78 @@ template <class R> V3<R> DisneyEval(V3<R> dir, const HitData<R>& h, V3<R> L) {  
     return brdf;
 }
 
+// Derivatives of DisneyEval (above) with respect to k = albedo.x, albedo.y,
+// albedo.z, roughness, metallic, specular of HitData (the effective values,
+// after generateHitData's pow 2.2), written out analytically from the same
+// expression; df[k] is the RGB derivative.  Not part of the reference (it has
+// no adjoint); pinned against central differences of DisneyEval.
+template <class R> void DisneyEvalGrad(V3<R> dir, const HitData<R>& h, V3<R> L, V3<R>* df) {
+    for (int k = 0; k < 6; k++) df[k] = V3<R>((R)0);
+    V3<R> V = neg(dir);
+    V3<R> T, B;
+    createBasis(h.normal, T, B);
+    V3<R> H = (L + V).normalized();
+    R NDotL = std::fabs(dot(h.normal, L));
+    R NDotV = std::fabs(dot(h.normal, V));
+    R NDotH = std::fabs(dot(h.normal, H));
+    R LDotH = std::fabs(dot(L, H));
+    if (!(h.transmission < 1.0 && dot(h.normal, L) > 0.0 && dot(h.normal, V) > 0.0)) return;
+    const R pi = piR<R>();
+    V3<R> C = h.albedo;
+    const R wl[3] = {(R)0.3, (R)0.6, (R)0.1};
+    R lum = wl[0] * C.x + wl[1] * C.y + wl[2] * C.z;
+    V3<R> Ctint = lum > 0.0 ? C / lum : V3<R>((R)1.0);
+    V3<R> dCtint[3];
+    for (int k = 0; k < 3; k++) {
+        V3<R> e((R)(k == 0), (R)(k == 1), (R)(k == 2));
+        dCtint[k] = lum > 0.0 ? e / lum - C * (wl[k] / (lum * lum)) : V3<R>((R)0);
+    }
+    const R mu = h.metallic, rho = h.roughness, sig = h.specular;
+    V3<R> tintMix = V3<R>((R)1) + (Ctint - V3<R>((R)1)) * h.specularTint;
+    R S0 = sig * (R)0.08;
+    V3<R> Cspec0 = S0 * tintMix + (C - S0 * tintMix) * mu;
+    V3<R> Csheen = V3<R>((R)1) + (Ctint - V3<R>((R)1)) * h.sheenTint;
+    R FL = SchlickFresnel<R>(NDotL), FV = SchlickFresnel<R>(NDotV), FH = SchlickFresnel<R>(LDotH);
+    R Fd90 = 0.5 + 2.0 * LDotH * LDotH * rho;
+    R A = 1 + FL * (Fd90 - 1), Bd = 1 + FV * (Fd90 - 1);
+    R Fd = A * Bd, dFd = 2 * LDotH * LDotH * (FL * Bd + FV * A);
+    R Fss90 = LDotH * LDotH * rho;
+    R A2 = 1 + FL * (Fss90 - 1), B2 = 1 + FV * (Fss90 - 1);
+    R Fss = A2 * B2, dFss = LDotH * LDotH * (FL * B2 + FV * A2);
+    R k1 = 1.0 / (NDotL + NDotV) - 0.5;
+    R ss = 1.25 * (Fss * k1 + 0.5), dss = 1.25 * k1 * dFss;
+    R D0 = Fd + h.subsurface * (ss - Fd), dD0 = dFd + h.subsurface * (dss - dFd);
+    R aspect = std::sqrt((R)(1.0 - h.anisotropic * 0.9));
+    R ax = maxR<R>(0.001, rho / aspect), ay = maxR<R>(0.001, rho * aspect);
+    R dax = rho / aspect > 0.001 ? 1 / aspect : 0, day = rho * aspect > 0.001 ? aspect : 0;
+    R HX = dot(H, T), HY = dot(H, B);
+    R a = HX / ax, b = HY / ay, c = a * a + b * b + NDotH * NDotH;
+    R Ds = 1.0 / (pi * ax * ay * c * c);
+    R dDs = Ds * (-1 / ax + 4 * a * a / (c * ax)) * dax + Ds * (-1 / ay + 4 * b * b / (c * ay)) * day;
+    auto G = [&](R N, R X, R Y, R& g, R& dg) {
+        R q = std::sqrt((X * ax) * (X * ax) + (Y * ay) * (Y * ay) + N * N);
+        g = 1.0 / (N + q);
+        dg = -g * g * ((X * X * ax) * dax + (Y * Y * ay) * day) / q;
+    };
+    R GL, dGL, GV, dGV;
+    G(NDotL, dot(L, T), dot(L, B), GL, dGL);
+    G(NDotV, dot(V, T), dot(V, B), GV, dGV);
+    R Gs = GL * GV, dGs = dGL * GV + GL * dGV;
+    V3<R> Fs = Cspec0 + (V3<R>((R)1) - Cspec0) * FH;
+    V3<R> Fsheen = Csheen * (FH * h.sheen);
+    V3<R> diffusePart = C * (R)((1.0 / pi) * D0) + Fsheen;
+    R spec = Gs * Ds * (1 - FH);
+    for (int k = 0; k < 3; k++) {
+        V3<R> e((R)(k == 0), (R)(k == 1), (R)(k == 2));
+        V3<R> dCspec0 = S0 * h.specularTint * (1 - mu) * dCtint[k] + e * mu;
+        V3<R> dFsheen = dCtint[k] * (FH * h.sheen * h.sheenTint);
+        df[k] = (e * (R)((1.0 / pi) * D0) + dFsheen) * (R)(1 - mu) + dCspec0 * spec;
+    }
+    df[3] = C * (R)((1.0 / pi) * dD0 * (1 - mu)) + Fs * (R)(dGs * Ds + Gs * dDs);
+    df[4] = neg(diffusePart) + (C - S0 * tintMix) * spec;
+    df[5] = tintMix * ((R)0.08 * (1 - mu) * spec);
+}
+
 // ----------------------------------------------------------- texture -----
 struct Tex {
     int w = 1, h = 1;
@@ -521,6 +593,14 @@ template <class R> V3<R> texel(const Tex& t, int x, int y) {
     if (y < 0) y += t.h;
     const float* p = t.data + 3 * ((size_t)y * t.w + x);
     return V3<R>((R)p[0], (R)p[1], (R)p[2]);
+}
+// Linear index of the texel texel() reads for (x, y).
+inline size_t texelIndex(const Tex& t, int x, int y) {
+    x = (int)(t.xTile * (x + t.xOffset * t.w)) % t.w;
+    y = (int)(t.yTile * (y + t.yOffset * t.h)) % t.h;
+    if (x < 0) x += t.w;
+    if (y < 0) y += t.h;
+    return (size_t)y * t.w + x;
 }
 template <class R> V3<R> texUV(const Tex& t, R u, R v) { return texel<R>(t, (int)(u * t.w), (int)(v * t.h)); }
 template <class R> V3<R> texBilinear(const Tex& t, R u, R v) {   // Texture.hpp:114-135
@@ -555,6 +635,7 @@ struct SceneC {
     std::vector<Tex> texs;
     std::vector<float> cdf;
     float radianceSum = 0;
+    const el_material* evalMats = nullptr;   // or_render_params::eval_materials
 };
 
 template <class R>
@@ -605,8 +686,8 @@ Surf<R> surface(const SceneC& s, int tri, float t, float u_, float v_, V3<R> o, 
 
 // generateHitData (kernel.cu:54-119)
 template <class R>
-HitData<R> hitData(const SceneC& s, const Surf<R>& h) {
-    const el_material& m = s.v->materials[h.material];
+HitData<R> hitData(const SceneC& s, const Surf<R>& h, const el_material* mats = nullptr) {
+    const el_material& m = (mats ? mats : s.v->materials)[h.material];
     auto tex = [&](int id) -> const Tex* {
         return (id >= 0 && (size_t)id < s.texs.size()) ? &s.texs[id] : nullptr;
     };
@@ -659,10 +740,30 @@ Hit closest(const SceneC& s, V3<R> o, V3<R> d) {
     return refTraverse(*s.bvh, of, df);
 }
 
-// One path (kernel.cu:369-481).  Returns true if the sample is accepted.
+// Everything one path vertex computed, for the adjoint's replay.
 template <class R>
-bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_t sample, V3<R>& Lout,
-               uint64_t& rays, uint64_t& verts) {
+struct VertexRec {
+    int depth;
+    V3<R> d, T;                 // incoming direction, throughput before the vertex
+    Surf<R> sh;
+    HitData<R> hd;
+    V3<R> bdir, f;              // sampled direction and DisneyEval there
+    R bpdf, cosS;
+    V3<R> hdir, hval;           // HDRI sample direction and texel
+    R hpdf, cosH;
+    bool visH;
+    bool hasP, visP;
+    V3<R> ldir, lrad;           // light direction, radiance / dist^2
+    R ppdf, cosL;
+    R w1, w2, w3;
+    V3<R> Lincl;                // radiance gathered up to and including this vertex
+};
+
+// One path (kernel.cu:369-481), radiance before the clamp; `hook` sees every
+// vertex.
+template <class R, class Hook>
+void walkPath(const SceneC& s, const or_render_params& p, int x, int y, uint32_t sample, V3<R>& Lraw,
+              uint64_t& rays, uint64_t& verts, Hook&& hook) {
     const el_scene_view& v = *s.v;
     const el_camera& cam = v.camera;
     uint32_t W = cam.x_res, Hh = cam.y_res;
@@ -721,17 +822,22 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
         }
         Surf<R> sh = surface<R>(s, hit.tri, hit.t, hit.u, hit.v, o, d);
         HitData<R> hd = hitData<R>(s, sh);
+        // BRDF values from the evaluation materials when given (sampling,
+        // pdfs and MIS weights keep the scene's): the detached-sampling
+        // estimator whose derivative the adjoint computes
+        const HitData<R> he = s.evalMats ? hitData<R>(s, sh, s.evalMats) : hd;
         uint32_t r[4];
         pathBlock(k0, k1, idx, sample, 2 + i, r);
         V3<R> bdir = DisneySample<R>(d, hd, u01(r[0]), u01(r[1]), u01(r[2]));
 
         // shade (kernel.cu:339-358)
-        V3<R> f = DisneyEval<R>(d, hd, bdir);
+        V3<R> f = DisneyEval<R>(d, he, bdir);
         R bpdf = DisneyPdf<R>(d, hd, bdir);
         R r1 = u01(r[3]);
         // hdriLight (kernel.cu:234-257)
         R hpdf;
-        V3<R> Hl;
+        V3<R> Hl, vHdir, vHval, vLdir, vLrad;
+        bool vHvis = false, vPvis = false;
         {
             const Tex& e = s.env;
             int k = binarySearch(s.cdf.data(), (float)r1, e.w * e.h);   // HDRI::sample
@@ -751,7 +857,7 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
             rays++;
             bool occ = closest(s, so, sd).tri >= 0;
             V3<R> val = texUV<R>(e, iu, iv);
-            V3<R> fe = DisneyEval<R>(d, hd, nd);
+            V3<R> fe = DisneyEval<R>(d, he, nd);
             // HDRI::pdf((int)(iu*W), (int)(iv*H)) (HDRI.hpp:145-152), evaluated
             // whether or not the shadow ray is blocked
             int qx = (int)(iu * e.w), qy = (int)(iv * e.h);
@@ -759,6 +865,9 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
             R th = (((R)qy / (R)e.h)) * piR<R>();
             hpdf = ((dv.x + dv.y + dv.z) / (R)s.radianceSum) * (R)e.w * (R)e.h / (2.0 * piR<R>() * std::sin(th));
             Hl = occ ? V3<R>() : fe * std::fabs(dot(nd, hd.normal)) * val / hpdf;
+            vHdir = nd;
+            vHval = val;
+            vHvis = !occ;
         }
         // pointLight (kernel.cu:175-205)
         R ppdf = 0;
@@ -786,19 +895,62 @@ bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_
             } else {
                 occ = sh2.tri >= 0 && sh2.t < (float)(dist - (R)0.001);
             }
-            Pl = occ ? V3<R>() : (lr / (dist * dist)) * DisneyEval<R>(d, hd, nd) * std::fabs(dot(nd, hd.normal)) / ppdf;
+            Pl = occ ? V3<R>() : (lr / (dist * dist)) * DisneyEval<R>(d, he, nd) * std::fabs(dot(nd, hd.normal)) / ppdf;
+            vLdir = nd;
+            vLrad = lr / (dist * dist);
+            vPvis = !occ;
         }
         V3<R> B = hd.emission * (f * std::fabs(dot(bdir, hd.normal))) / bpdf;
         R w1 = hpdf / (hpdf + ppdf + bpdf);
         R w2 = ppdf / (hpdf + ppdf + bpdf);
         R w3 = bpdf / (hpdf + ppdf + bpdf);
+        VertexRec<R> rec;
+        rec.depth = i;
+        rec.d = d;
+        rec.T = T;
         L = L + T * (w1 * Hl + w2 * Pl + w3 * B);
         T = T * ((f * std::fabs(dot(bdir, hd.normal))) / bpdf);
         verts++;
+        rec.sh = sh;
+        rec.hd = hd;
+        rec.bdir = bdir;
+        rec.f = f;
+        rec.bpdf = bpdf;
+        rec.cosS = std::fabs(dot(bdir, hd.normal));
+        rec.hdir = vHdir;
+        rec.hval = vHval;
+        rec.hpdf = hpdf;
+        rec.cosH = std::fabs(dot(vHdir, hd.normal));
+        rec.visH = vHvis;
+        rec.hasP = nl > 0;
+        rec.visP = vPvis;
+        rec.ldir = vLdir;
+        rec.lrad = vLrad;
+        rec.ppdf = ppdf;
+        rec.cosL = std::fabs(dot(vLdir, hd.normal));
+        rec.w1 = w1;
+        rec.w2 = w2;
+        rec.w3 = w3;
+        rec.Lincl = L;
+        hook(rec);
         o = sh.position + bdir * (R)0.001;
         d = bdir.normalized();
     }
     (void)i;
+    Lraw = L;
+}
+
+struct NoHook {
+    template <class T>
+    void operator()(const T&) const {}
+};
+
+// One path (kernel.cu:369-481).  Returns true if the sample is accepted.
+template <class R>
+bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_t sample, V3<R>& Lout,
+               uint64_t& rays, uint64_t& verts) {
+    V3<R> L;
+    walkPath<R>(s, p, x, y, sample, L, rays, verts, NoHook());
     L = V3<R>(clampR<R>(L.x, 0, 10), clampR<R>(L.y, 0, 10), clampR<R>(L.z, 0, 10));
     if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) return false;
     Lout = L;
@@ -835,6 +987,126 @@ void renderT(const SceneC& s, const or_render_params& p, double* sum, double* su
         rays += lr;
         verts += lv;
     });
+}
+
+// Path-replay adjoint (Vicini, Speierer, Jakob 2021) of the estimator above
+// with detached sampling: sampled directions, lobe choices, pdfs and MIS
+// weights are held; the derivative flows through DisneyEval at the sampled,
+// HDRI and light directions.  Per accepted sample with adjoint weight a
+// (dLoss/dI_pixel / accepted samples, zeroed on clamped channels):
+//   g += a . [ (T w3 Le cosS/bpdf + L_after / f) dF_s
+//            + T w1 env cosH/hpdf dF_h [visible] + T w2 rad cosL/ppdf dF_l [visible] ]
+// with L_after = radiance gathered after the vertex (L_total - L_incl).
+template <class R>
+void adjointT(const SceneC& s, const or_render_params& rp, const or_adjoint_params& p, const float* dimage,
+              double* dmat, double* dtex, double* image) {
+    const el_scene_view& v = *s.v;
+    const el_camera& cam = v.camera;
+    const int W = (int)cam.x_res, Hh = (int)cam.y_res, npx = W * Hh;
+    const int nthreads = std::max(1, rp.threads > 0 ? rp.threads : (int)std::thread::hardware_concurrency());
+    const int M = (int)v.material_count;
+    const Tex* gtex = (p.grad_texture >= 0 && (size_t)p.grad_texture < s.texs.size()) ? &s.texs[p.grad_texture] : nullptr;
+    const size_t ntex = gtex ? (size_t)gtex->w * gtex->h * 3 : 0;
+    std::vector<std::vector<double>> gm(nthreads, std::vector<double>((size_t)M * 6, 0.0));
+    std::vector<std::vector<double>> gt(nthreads, std::vector<double>(ntex, 0.0));
+    std::atomic<int> next(0);
+    auto work = [&](int tid) {
+        std::vector<V3<R>> Ls(p.spp);
+        std::vector<char> ok(p.spp);
+        for (;;) {
+            int k = next.fetch_add(1);
+            if (k >= npx) break;
+            int x = k % W, y = k / W;
+            uint32_t idx = (uint32_t)(W * (Hh - y - 1) + x);
+            uint64_t rays = 0, verts = 0;
+            int n = 0;
+            double mean[3] = {0, 0, 0};
+            for (uint32_t j = 0; j < p.spp; j++) {
+                walkPath<R>(s, rp, x, y, p.s0 + j, Ls[j], rays, verts, NoHook());
+                V3<R> c(clampR<R>(Ls[j].x, 0, 10), clampR<R>(Ls[j].y, 0, 10), clampR<R>(Ls[j].z, 0, 10));
+                ok[j] = !(std::isnan(c.x) || std::isnan(c.y) || std::isnan(c.z));
+                if (ok[j]) {
+                    n++;
+                    mean[0] += (double)c.x;
+                    mean[1] += (double)c.y;
+                    mean[2] += (double)c.z;
+                }
+            }
+            if (image)
+                for (int c = 0; c < 3; c++) image[3 * (size_t)idx + c] = n ? mean[c] / n : 0.0;
+            if (!n) continue;
+            const float* di = dimage + 3 * (size_t)idx;
+            for (uint32_t j = 0; j < p.spp; j++) {
+                if (!ok[j]) continue;
+                const V3<R> Lt = Ls[j];
+                R a[3];
+                const R Lc[3] = {Lt.x, Lt.y, Lt.z};
+                for (int c = 0; c < 3; c++) a[c] = (Lc[c] >= 0 && Lc[c] <= 10) ? (R)di[c] / (R)n : (R)0;
+                if (a[0] == 0 && a[1] == 0 && a[2] == 0) continue;
+                auto hook = [&](const VertexRec<R>& rec) {
+                    V3<R> dfs[6], dfh[6], dfl[6];
+                    DisneyEvalGrad<R>(rec.d, rec.hd, rec.bdir, dfs);
+                    DisneyEvalGrad<R>(rec.d, rec.hd, rec.hdir, dfh);
+                    for (int q = 0; q < 6; q++) dfl[q] = V3<R>((R)0);
+                    if (rec.hasP) DisneyEvalGrad<R>(rec.d, rec.hd, rec.ldir, dfl);
+                    V3<R> La = Lt - rec.Lincl;
+                    V3<R> cS = rec.T * rec.hd.emission * (rec.w3 * rec.cosS / rec.bpdf);
+                    V3<R> cH = rec.visH ? rec.T * rec.hval * (rec.w1 * rec.cosH / rec.hpdf) : V3<R>((R)0);
+                    V3<R> cP = (rec.hasP && rec.visP) ? rec.T * rec.lrad * (rec.w2 * rec.cosL / rec.ppdf) : V3<R>((R)0);
+                    const R fa[3] = {rec.f.x, rec.f.y, rec.f.z}, la[3] = {La.x, La.y, La.z};
+                    R ratio[3];
+                    for (int c = 0; c < 3; c++) ratio[c] = fa[c] != 0 ? la[c] / fa[c] : (R)0;
+                    const R cs[3] = {cS.x, cS.y, cS.z}, ch[3] = {cH.x, cH.y, cH.z}, cp[3] = {cP.x, cP.y, cP.z};
+                    R g[6];
+                    for (int q = 0; q < 6; q++) {
+                        const R ds[3] = {dfs[q].x, dfs[q].y, dfs[q].z}, dh[3] = {dfh[q].x, dfh[q].y, dfh[q].z},
+                                dl[3] = {dfl[q].x, dfl[q].y, dfl[q].z};
+                        R acc = 0;
+                        for (int c = 0; c < 3; c++) acc += a[c] * ((cs[c] + ratio[c]) * ds[c] + ch[c] * dh[c] + cp[c] * dl[c]);
+                        g[q] = acc;
+                    }
+                    const int mat = rec.sh.material;
+                    const el_material& m = v.materials[mat];
+                    double* G = gm[tid].data() + (size_t)mat * 6;
+                    const R e = (R)2.2f;
+                    if (m.albedo_tex < 0) {
+                        for (int q = 0; q < 3; q++) G[q] += (double)g[q];
+                    } else if (gtex && m.albedo_tex == p.grad_texture) {
+                        const Tex& t = *gtex;
+                        double* D = gt[tid].data();
+                        if (t.filter == 1) {
+                            R xx = rec.sh.tu * t.w, yy = rec.sh.tv * t.h;
+                            R t1x = std::floor(xx), t1y = std::floor(yy);
+                            R fx = xx - t1x, fy = yy - t1y;
+                            const int cx[4] = {(int)t1x, (int)t1x + 1, (int)t1x, (int)t1x + 1};
+                            const int cy[4] = {(int)t1y, (int)t1y, (int)t1y + 1, (int)t1y + 1};
+                            const R wgt[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
+                            for (int c4 = 0; c4 < 4; c4++) {
+                                size_t ti = texelIndex(t, cx[c4], cy[c4]);
+                                for (int q = 0; q < 3; q++) D[3 * ti + q] += (double)(g[q] * wgt[c4]);
+                            }
+                        } else {
+                            size_t ti = texelIndex(t, (int)(rec.sh.tu * t.w), (int)(rec.sh.tv * t.h));
+                            for (int q = 0; q < 3; q++) D[3 * ti + q] += (double)g[q];
+                        }
+                    }
+                    if (m.roughness_tex < 0) G[3] += (double)(g[3] * e * std::pow((R)m.roughness, e - 1));
+                    if (m.metallic_tex < 0) G[4] += (double)(g[4] * e * std::pow((R)m.metallic, e - 1));
+                    G[5] += (double)g[5];
+                };
+                V3<R> Lr;
+                walkPath<R>(s, rp, x, y, p.s0 + j, Lr, rays, verts, hook);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++) pool.emplace_back(work, t);
+    for (auto& t : pool) t.join();
+    for (int t = 0; t < nthreads; t++) {
+        for (size_t i = 0; i < (size_t)M * 6; i++) dmat[i] += gm[t][i];
+        if (dtex)
+            for (size_t i = 0; i < ntex; i++) dtex[i] += gt[t][i];
+    }
 }
 
 }  // namespace
@@ -1071,6 +1343,7 @@ int or_render(const el_scene_view* v, void* refbvh, const or_render_params* p, d
     size_t L = (size_t)v->hdri.width * v->hdri.height;
     s.cdf.assign(L + 1, 0.f);
     or_hdri_cdf(v->hdri.data, v->hdri.width, v->hdri.height, s.cdf.data(), &s.radianceSum);
+    s.evalMats = p->eval_materials;
     std::atomic<uint64_t> r{0}, vv{0};
     if (p->precision == 1)
         renderT<double>(s, *p, sum, sumsq, count, r, vv);
@@ -1078,6 +1351,50 @@ int or_render(const el_scene_view* v, void* refbvh, const or_render_params* p, d
         renderT<float>(s, *p, sum, sumsq, count, r, vv);
     if (rays) *rays = r.load();
     if (vertices) *vertices = vv.load();
+    return 0;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+void or_disney_eval_grad(const double* hd, const double* dir, const double* L, double* out18) {
+    V3<double> df[6];
+    DisneyEvalGrad<double>(V3<double>(dir[0], dir[1], dir[2]), hdD(hd), V3<double>(L[0], L[1], L[2]), df);
+    for (int k = 0; k < 6; k++) {
+        out18[3 * k] = df[k].x;
+        out18[3 * k + 1] = df[k].y;
+        out18[3 * k + 2] = df[k].z;
+    }
+}
+
+int or_adjoint(const el_scene_view* v, void* refbvh, const or_adjoint_params* p, const float* dimage, double* dmat,
+               double* dtex, double* image) {
+    if (!v || !refbvh || !p || !dimage || !dmat) return -1;
+    SceneC s;
+    s.v = v;
+    s.bvh = (const RefBVH*)refbvh;
+    s.env = texOf(v->hdri);
+    for (uint32_t i = 0; i < v->texture_count; i++) s.texs.push_back(texOf(v->textures[i]));
+    size_t L = (size_t)v->hdri.width * v->hdri.height;
+    s.cdf.assign(L + 1, 0.f);
+    or_hdri_cdf(v->hdri.data, v->hdri.width, v->hdri.height, s.cdf.data(), &s.radianceSum);
+    or_render_params rp{};
+    rp.spp = p->spp;
+    rp.s0 = p->s0;
+    rp.depth = p->depth;
+    rp.seed = p->seed;
+    rp.precision = p->precision;
+    rp.shadow_mode = p->shadow_mode;
+    rp.threads = p->threads;
+    rp.x0 = 0;
+    rp.x1 = (int)v->camera.x_res;
+    rp.y0 = 0;
+    rp.y1 = (int)v->camera.y_res;
+    if (p->precision == 1)
+        adjointT<double>(s, rp, *p, dimage, dmat, dtex, image);
+    else
+        adjointT<float>(s, rp, *p, dimage, dmat, dtex, image);
     return 0;
 }
 

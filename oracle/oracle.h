@@ -85,10 +85,35 @@ typedef struct {
     uint64_t seed;
     int precision, shadow_mode, threads;
     int x0, x1, y0, y1;
+    /* NULL, or materials whose BRDF values DisneyEval uses while sampling,
+     * pdfs and MIS weights keep the scene's: the detached-sampling estimator
+     * whose derivative or_adjoint computes (test use only). */
+    const el_material* eval_materials;
 } or_render_params;
 
 int or_render(const el_scene_view* scene, void* refbvh, const or_render_params* p, double* sum, double* sumsq,
               uint32_t* count, uint64_t* rays, uint64_t* vertices);
+
+/* d DisneyEval / d (albedo.x, albedo.y, albedo.z, roughness, metallic,
+ * specular) of HitData (effective values), FP64, analytic: out18[3k + c].
+ * Not part of the reference (it has no adjoint). */
+void or_disney_eval_grad(const double* hd, const double* dir, const double* L, double* out18);
+
+/* Path-replay adjoint of the estimator (detached sampling): gradients of
+ * sum_p sum_c dimage[idx][c] * I[idx][c] with respect to every material's
+ * (albedo r, g, b, roughness, metallic, specular) - raw values, before the
+ * pow 2.2 of generateHitData - added into dmat[M][6], and to the texels of
+ * texture `grad_texture` added into dtex[h*w*3] (may be NULL).  I is the mean
+ * of the accepted samples s0 .. s0+spp-1; image (optional) receives it.
+ * dimage / image are indexed by the reference's idx = W*(H-y-1)+x. */
+typedef struct {
+    uint32_t spp, s0, depth;
+    uint64_t seed;
+    int precision, shadow_mode, threads;
+    int grad_texture;
+} or_adjoint_params;
+int or_adjoint(const el_scene_view* scene, void* refbvh, const or_adjoint_params* p, const float* dimage,
+               double* dmat, double* dtex, double* image);
 
 #ifdef __cplusplus
 }

@@ -17,10 +17,16 @@ ORACLE_SO = ROOT / "oracle" / "liboracle.so"
 _lib = None
 
 
+class or_adjoint_params(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("s0", C.c_uint32), ("depth", C.c_uint32), ("seed", C.c_uint64),
+                ("precision", C.c_int), ("shadow_mode", C.c_int), ("threads", C.c_int), ("grad_texture", C.c_int)]
+
+
 class or_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("s0", C.c_uint32), ("depth", C.c_uint32), ("seed", C.c_uint64),
                 ("precision", C.c_int), ("shadow_mode", C.c_int), ("threads", C.c_int),
-                ("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int)]
+                ("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int),
+                ("eval_materials", C.c_void_p)]
 
 
 def lib():
@@ -57,6 +63,9 @@ def lib():
         L.or_render.argtypes = [P, P, C.POINTER(or_render_params), P, P, P, C.POINTER(C.c_uint64),
                                 C.POINTER(C.c_uint64)]
         L.or_render.restype = C.c_int
+        L.or_disney_eval_grad.argtypes = [P, P, P, P]
+        L.or_adjoint.argtypes = [P, P, C.POINTER(or_adjoint_params), P, P, P, P]
+        L.or_adjoint.restype = C.c_int
         _lib = L
     return _lib
 
@@ -147,6 +156,14 @@ def disney_eval(hd, direction, L, precision=1):
     return out
 
 
+def disney_eval_grad(hd, direction, L):
+    """FP64 d f / d (albedo r, g, b, roughness, metallic, specular): [6][3]."""
+    out = np.zeros(18)
+    h, d, l = _f64(hd), _f64(direction), _f64(L)
+    lib().or_disney_eval_grad(h.ctypes.data, d.ctypes.data, l.ctypes.data, out.ctypes.data)
+    return out.reshape(6, 3)
+
+
 def disney_pdf(hd, direction, L, precision=1):
     h, d, l = _f64(hd), _f64(direction), _f64(L)
     return lib().or_disney_pdf(h.ctypes.data, d.ctypes.data, l.ctypes.data, precision)
@@ -205,9 +222,12 @@ def camera_ray(cam, x, y, r5):
     return o, d
 
 
-def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1, threads=0, window=None):
+def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1, threads=0, window=None,
+           eval_materials=None):
     """Oracle render of pixels `window` = (x0, x1, y0, y1) (kernel x, y).
 
+    eval_materials: optional ctypes array of el_material whose BRDF values
+    are used while sampling, pdfs and MIS weights keep the scene's.
     Returns (sum[H,W,3], sumsq[H,W,3], count[H,W], rays, vertices) in output
     layout (row 0 = top)."""
     cam = scene.view.camera
@@ -217,6 +237,7 @@ def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1
     p.spp, p.s0, p.depth, p.seed = spp, s0, depth, seed
     p.precision, p.shadow_mode, p.threads = precision, shadow_mode, threads
     p.x0, p.x1, p.y0, p.y1 = x0, x1, y0, y1
+    p.eval_materials = C.cast(eval_materials, C.c_void_p) if eval_materials is not None else None
     s = np.zeros((H, W, 3))
     ss = np.zeros((H, W, 3))
     cnt = np.zeros((H, W), np.uint32)
@@ -226,3 +247,25 @@ def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1
     if rc != 0:
         raise ValueError("or_render rejected its arguments")
     return s, ss, cnt, rays.value, verts.value
+
+
+def adjoint(scene, refbvh, dimage, spp, depth=5, seed=0, s0=0, precision=1, shadow_mode=1, threads=0,
+            grad_texture=-1):
+    """Path-replay adjoint on the CPU: returns (dmat [M][6], dtex [h][w][3] or None, image [H][W][3])
+    for the loss sum(dimage * I); dimage and image in the reference's idx order
+    (row 0 = top, as el_read_passes)."""
+    v = scene.view
+    H, W = v.camera.y_res, v.camera.x_res
+    di = np.ascontiguousarray(dimage, np.float32).reshape(H * W * 3)
+    dmat = np.zeros((v.material_count, 6))
+    dtex = None
+    if grad_texture >= 0:
+        t = v.textures[grad_texture]
+        dtex = np.zeros((t.height, t.width, 3))
+    img = np.zeros((H, W, 3))
+    p = or_adjoint_params(spp, s0, depth, seed, precision, shadow_mode, threads, grad_texture)
+    rc = lib().or_adjoint(C.byref(v), refbvh.h, C.byref(p), di.ctypes.data, dmat.ctypes.data,
+                          dtex.ctypes.data if dtex is not None else None, img.ctypes.data)
+    if rc != 0:
+        raise ValueError("or_adjoint rejected its arguments")
+    return dmat, dtex, img

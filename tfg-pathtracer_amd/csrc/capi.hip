@@ -96,7 +96,11 @@ struct el_ctx {
     DevBuf tRays, tTmax, tTri, tT, tUV, tOcc;   // scratch for host-pointer ray queries
 
     uint32_t sampleBase = 0;
+    uint64_t accSeed = 0;          // seed / depth of the renders in the accumulation
+    uint32_t accDepth = 0;
+    bool accMixed = false;         // renders since el_clear used different seeds or depths
     uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
+    DevBuf aImage, aMat, aTex;     // adjoint image, FP64 material and FP32 texel gradients
     el_stats stats{};
     bool aovOn = false;
 
@@ -211,6 +215,7 @@ static int clearAccum(el_ctx* c) {
     EL_HIP(c, hipMemsetAsync(c->dCounts.p, 0, npx * sizeof(uint32_t), c->stream));
     if (c->dAov.p) EL_HIP(c, hipMemsetAsync(c->dAov.p, 0, c->dAov.n, c->stream));
     c->sampleBase = 0;
+    c->accMixed = false;
     c->totalRays = c->totalVerts = c->totalPaths = 0;
     EL_HIP(c, hipStreamSynchronize(c->stream));
     return EL_OK;
@@ -565,6 +570,12 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     unsigned long long h[kCounterWords];
     EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
     EL_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->sampleBase == 0) {
+        c->accSeed = p->seed;
+        c->accDepth = depth;
+    } else if (c->accSeed != p->seed || c->accDepth != depth) {
+        c->accMixed = true;
+    }
     c->sampleBase += p->spp;
     c->totalRays += tally(h, kStatRays);
     c->totalVerts += tally(h, kStatVerts);
@@ -581,6 +592,68 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     c->stats.node_fetches = tally(h, kStatNodes);
     c->stats.tri_tests = tally(h, kStatTris);
     if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack or queue overflow (results incomplete)");
+    return EL_OK;
+}
+
+extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* dimage, float* dmat, float* dtex,
+                          int devp) {
+    if (!c || !p || !dimage || !dmat) return fail(c, EL_ERR_ARG, "null argument");
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (c->sampleBase == 0) return fail(c, EL_ERR_STATE, "nothing rendered since el_clear: no image to differentiate");
+    if (c->accMixed) return fail(c, EL_ERR_STATE, "the accumulation mixes renders of different seeds or depths");
+    if (c->accDepth > (uint32_t)dev::kAdjMaxDepthHost) return fail(c, EL_ERR_RANGE, "adjoint supports max_depth <= 16");
+    if (p->block != 0 && (p->block % 64 != 0 || p->block > (uint32_t)kMaxBlock))
+        return fail(c, EL_ERR_ARG, "block must be a multiple of 64, at most 256");
+    const int gtex = p->grad_texture;
+    if (gtex >= (int)c->texs.size()) return fail(c, EL_ERR_ARG, "grad_texture out of range");
+    if (gtex >= 0 && !dtex) return fail(c, EL_ERR_ARG, "grad_texture set but no texture gradient buffer");
+    EL_HIP(c, hipSetDevice(c->device));
+    const size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+    const size_t M = c->mats.size();
+    const size_t ntex = gtex >= 0 ? (size_t)c->texs[gtex].w * c->texs[gtex].h * 3 : 0;
+    const hipMemcpyKind h2d = devp ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    EL_HIP(c, c->aImage.ensure(npx * 3 * sizeof(float)));
+    EL_HIP(c, hipMemcpyAsync(c->aImage.p, dimage, npx * 3 * sizeof(float), h2d, c->stream));
+    EL_HIP(c, c->aMat.ensure(std::max<size_t>(1, M) * 6 * sizeof(double)));
+    EL_HIP(c, hipMemsetAsync(c->aMat.p, 0, M * 6 * sizeof(double), c->stream));
+    if (ntex) {
+        EL_HIP(c, c->aTex.ensure(ntex * sizeof(float)));
+        EL_HIP(c, hipMemsetAsync(c->aTex.p, 0, ntex * sizeof(float), c->stream));
+    }
+    unsigned long long* ctr = c->dCounters.as<unsigned long long>();
+    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, c->stream));
+    LaunchCfg lc = cfgOf(c);
+    if (p->block) lc.block = p->block;
+    RenderArgs a{};
+    a.k0 = (uint32_t)c->accSeed;
+    a.k1 = (uint32_t)(c->accSeed >> 32);
+    a.scene = 0;
+    a.maxDepth = c->accDepth;
+    a.sample = 0;
+    hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
+    if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
+    EL_HIP(c, hipEventRecord(e0, c->stream));
+    EL_HIP(c, launchAdjoint(lc, c->ds, a, c->sampleBase, c->aImage.as<float>(), c->dCounts.as<uint32_t>(), gtex,
+                            c->aMat.as<double>(), ntex ? c->aTex.as<float>() : nullptr, ctr));
+    EL_HIP(c, hipEventRecord(e1, c->stream));
+    std::vector<double> hm(M * 6);
+    EL_HIP(c, hipMemcpyAsync(hm.data(), c->aMat.p, M * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long h[kCounterWords];
+    EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    EL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+    std::vector<float> fm(M * 6);
+    for (size_t i = 0; i < M * 6; i++) fm[i] = (float)hm[i];
+    if (devp)
+        EL_HIP(c, hipMemcpy(dmat, fm.data(), fm.size() * sizeof(float), hipMemcpyHostToDevice));
+    else
+        std::memcpy(dmat, fm.data(), fm.size() * sizeof(float));
+    if (ntex)
+        EL_HIP(c, hipMemcpy(dtex, c->aTex.p, ntex * sizeof(float), devp ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    c->stats.adjoint_ms = ms;
+    c->stats.adjoint_rays = tally(h, kStatRays);
+    if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack overflow (gradients incomplete)");
     return EL_OK;
 }
 

@@ -210,6 +210,13 @@ struct VertexOut {
     bool hasP;
     v3 newT;        // throughput after the bounce
     v3 nextO, nextD;
+    // primal terms the adjoint replay reuses
+    v3 bdir, f;     // sampled direction, DisneyEval there
+    float bpdf, w1, w2, w3;
+    v3 hdir, hval;  // HDRI sample direction and texel
+    float hpdf;
+    v3 ldir, lrad;  // light direction, radiance / dist^2
+    float ppdf;
 };
 
 // One path vertex: DisneySample, DisneyEval/Pdf, HDRI and point-light NEE
@@ -243,6 +250,11 @@ __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, 
         o.pDir = normalized(nd);
         o.pTmax = dist - 0.001f;
         P = (lr / (dist * dist)) * disneyEval(d, hd, nd) * fabsf(dot(nd, hd.normal)) / ppdf;
+        o.ldir = nd;
+        o.lrad = lr / (dist * dist);
+    } else {
+        o.ldir = mk(0, 0, 1);
+        o.lrad = mk(0, 0, 0);
     }
     v3 fc = f * fabsf(dot(bdir, hd.normal));
     v3 B = hd.emission * fc / bpdf;
@@ -254,6 +266,16 @@ __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, 
     o.newT = T * (fc / bpdf);
     o.nextO = sh.position + bdir * 0.001f;
     o.nextD = normalized(bdir);
+    o.bdir = bdir;
+    o.f = f;
+    o.bpdf = bpdf;
+    o.w1 = w1;
+    o.w2 = w2;
+    o.w3 = w3;
+    o.hdir = es.dir;
+    o.hval = es.value;
+    o.hpdf = hpdf;
+    o.ppdf = ppdf;
     return o;
 }
 
@@ -263,6 +285,68 @@ __device__ __forceinline__ v3 clamp3(v3 a) {
 }
 
 // ------------------------------------------------------------------ plain --
+// What one path vertex of walkPath computed, for hooks (AOV capture, adjoint).
+struct PathVertex {
+    int depth;
+    v3 d, T;             // incoming direction, throughput before the vertex
+    const SurfaceHit* sh;
+    const HitData* hd;
+    const VertexOut* vo;
+    bool visH, visP;
+    v3 Lincl;            // radiance gathered up to and including this vertex
+};
+
+// One path of renderingKernel (kernel.cu:369-481) for pixel (x, y), sample
+// `sample`: camera ray, up to maxDepth vertices with DisneySample / shade and
+// both shadow queries, escape to the environment.  Returns the radiance
+// before the clamp; `hook` sees every shaded vertex; `verts` counts them.
+template <int W, class Hook>
+__device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, uint32_t x, uint32_t y,
+                                       uint32_t idx, uint32_t sample, TravStack& st, TraceCounters& cnt,
+                                       uint32_t& rays, uint32_t& verts, Hook&& hook) {
+    PathRng rng{a.k0, a.k1, idx, sample, a.scene};
+    uint4 c = rng.block(0);
+    uint32_t c5 = rng.block(1).x;
+    v3 o, d;
+    cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
+    v3 L = mk(0, 0, 0), T = mk(1, 1, 1);
+    int i;
+    for (i = 0; i < (int)a.maxDepth; i++) {
+        rays++;
+        TraceResult h = traceRay<W, false, false>(s, o.x, o.y, o.z, d.x, d.y, d.z, __int_as_float(0x7f800000), st, cnt);
+        if (h.tri < 0) {
+            L = L + envLookup(s, d) * T;
+            break;
+        }
+        SurfaceHit sh = surfaceAt(s, h.tri, h.t, h.u, h.v, o, d);
+        HitData hd = hitData(s, sh);
+        uint4 rb = rng.block(2 + i);
+        uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
+        VertexOut vo = shadeVertex(s, d, T, sh, hd, r);
+        rays++;
+        TraceResult hs = traceRay<W, true, false>(s, vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, vo.hDir.x, vo.hDir.y,
+                                                  vo.hDir.z, __int_as_float(0x7f800000), st, cnt);
+        v3 add = vo.base;
+        const bool visH = hs.tri < 0;
+        if (visH) add = add + vo.contribH;
+        bool visP = false;
+        if (vo.hasP) {
+            rays++;
+            TraceResult hp = traceRay<W, true, false>(s, vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, vo.pDir.x, vo.pDir.y,
+                                                      vo.pDir.z, vo.pTmax, st, cnt);
+            visP = hp.tri < 0;
+            if (visP) add = add + vo.contribP;
+        }
+        L = L + add;
+        hook(PathVertex{i, d, T, &sh, &hd, &vo, visH, visP, L});
+        T = vo.newT;
+        o = vo.nextO;
+        d = vo.nextD;
+    }
+    verts += (uint32_t)i;
+    return L;
+}
+
 template <int W>
 EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restrict__ accum, uint32_t* __restrict__ counts,
                         float4* __restrict__ aov, int* gstack, uint32_t gthreads,
@@ -279,51 +363,16 @@ EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restric
         uint32_t x = (tile % tilesX) * 16 + (in & 15), y = (tile / tilesX) * 16 + (in >> 4);
         if (x >= W_ || y >= H_) continue;
         uint32_t idx = W_ * (H_ - y - 1) + x;
-        PathRng rng{a.k0, a.k1, idx, a.sample, a.scene};
-        uint4 c = rng.block(0);
-        uint32_t c5 = rng.block(1).x;
-        v3 o, d;
-        cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
-        v3 L = mk(0, 0, 0), T = mk(1, 1, 1);
         v3 an = mk(0, 0, 0), at = mk(0, 0, 0), ab = mk(0, 0, 0);
-        int i;
         paths++;
-        for (i = 0; i < (int)a.maxDepth; i++) {
-            rays++;
-            TraceResult h = traceRay<W, false, false>(s, o.x, o.y, o.z, d.x, d.y, d.z,
-                                                       __int_as_float(0x7f800000), st, cnt);
-            if (h.tri < 0) {
-                L = L + envLookup(s, d) * T;
-                break;
+        v3 L = walkPath<W>(s, a, x, y, idx, a.sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
+            if (pv.depth == 0) {
+                an = pv.sh->normal;
+                at = pv.sh->tangent;
+                ab = pv.sh->bitangent;
             }
-            SurfaceHit sh = surfaceAt(s, h.tri, h.t, h.u, h.v, o, d);
-            HitData hd = hitData(s, sh);
-            uint4 rb = rng.block(2 + i);
-            uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
-            VertexOut vo = shadeVertex(s, d, T, sh, hd, r);
-            rays++;
-            TraceResult hs = traceRay<W, true, false>(s, vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, vo.hDir.x, vo.hDir.y,
-                                                      vo.hDir.z, __int_as_float(0x7f800000), st, cnt);
-            v3 add = vo.base;
-            if (hs.tri < 0) add = add + vo.contribH;
-            if (vo.hasP) {
-                rays++;
-                TraceResult hp = traceRay<W, true, false>(s, vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, vo.pDir.x,
-                                                          vo.pDir.y, vo.pDir.z, vo.pTmax, st, cnt);
-                if (hp.tri < 0) add = add + vo.contribP;
-            }
-            L = L + add;
-            if (i == 0) {
-                an = sh.normal;
-                at = sh.tangent;
-                ab = sh.bitangent;
-            }
-            T = vo.newT;
-            o = vo.nextO;
-            d = vo.nextD;
-        }
+        });
         overflow |= st.overflow;
-        verts += (uint32_t)i;
         L = clamp3(L);
         if (!anyNan(L)) {
             float4 acc = accum[idx];
@@ -334,6 +383,157 @@ EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restric
                 A[0] = make_float4(A[0].x + an.x, A[0].y + an.y, A[0].z + an.z, 0);
                 A[1] = make_float4(A[1].x + at.x, A[1].y + at.y, A[1].z + at.z, 0);
                 A[2] = make_float4(A[2].x + ab.x, A[2].y + ab.y, A[2].z + ab.z, 0);
+            }
+        }
+    }
+    const int slots[4] = {kStatRays, kStatVerts, kStatPaths, kStatOverflow};
+    const uint32_t vals[4] = {rays, verts, paths, overflow};
+    blockFlush<4>(counters, slots, vals);
+}
+
+// ---------------------------------------------------------------- adjoint --
+// Path-replay backpropagation (Vicini, Speierer, Jakob, "Path Replay
+// Backpropagation", SIGGRAPH 2021) of the estimator above, with detached
+// sampling: the sampled directions, lobe choices, pdfs and MIS weights of a
+// path are held; the derivative flows through DisneyEval at the sampled, HDRI
+// and light directions.  Each thread owns one (pixel, sample): it walks the
+// path once for its radiance L (the primal), then replays it from the same
+// Philox counters and, per vertex, adds
+//   a . [ (T w3 Le cosS/bpdf + L_after / f) dF_s
+//       + T w1 env cosH/hpdf dF_h [HDRI visible] + T w2 rad cosL/ppdf dF_l [light visible] ]
+// with a = dLoss/dI_pixel / accepted samples (0 on clamped channels) and
+// L_after = L - (radiance up to and including the vertex).  Nothing of the
+// path is stored: memory is the adjoint image, the counts and the gradients.
+// Per-vertex records (material, 6 gradients) are kept in registers until
+// the path ends; the wave then reduces them per material (ballot + shuffle
+// sums) and one lane per material adds them with an FP64 atomic.  Texel
+// gradients of the selected texture go straight to FP32 atomics.
+constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
+
+template <int W>
+EL_KERNEL(EL_WPE_PLAIN) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
+                                       const uint32_t* __restrict__ counts, int gradTex, double* __restrict__ dmat,
+                                       float* __restrict__ dtex, int* gstack, uint32_t gthreads,
+                                       unsigned long long* counters) {
+    TravStack st = makeStack(s, gstack, gthreads);
+    TraceCounters cnt{0, 0};
+    const uint32_t W_ = s.cam.xRes, H_ = s.cam.yRes;
+    const uint32_t tilesX = (W_ + 15) / 16;
+    const uint32_t perSample = tilesX * ((H_ + 15) / 16) * 256;
+    const uint64_t total = (uint64_t)perSample * spp;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t rays = 0, verts = 0, paths = 0, overflow = 0;
+    const uint32_t lane = __lane_id();
+    // uniform trip count: every lane of a wave runs the same iterations
+    const uint64_t base0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    for (uint64_t wb = base0; wb < total; wb += stride) {
+        const uint64_t w = wb + lane;
+        int nrec = 0;
+        int recMat[kAdjMaxDepth];
+        float recG[kAdjMaxDepth][6];
+        if (w < total) {
+            const uint32_t sample = a.sample + (uint32_t)(w / perSample);
+            const uint32_t pw = (uint32_t)(w % perSample);
+            const uint32_t tile = pw >> 8, in = pw & 255;
+            const uint32_t x = (tile % tilesX) * 16 + (in & 15), y = (tile / tilesX) * 16 + (in >> 4);
+            if (x < W_ && y < H_) {
+                const uint32_t idx = W_ * (H_ - y - 1) + x;
+                const uint32_t n = counts[idx];
+                uint32_t vdummy = 0;
+                const v3 Lt = walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, vdummy, [](const PathVertex&) {});
+                paths++;
+                const v3 Lc = clamp3(Lt);
+                float ad[3] = {0, 0, 0};
+                if (n && !anyNan(Lc)) {
+                    const float* di = dimage + 3 * (uint64_t)idx;
+                    const float Lr[3] = {Lt.x, Lt.y, Lt.z};
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) ad[ch] = (Lr[ch] >= 0.f && Lr[ch] <= 10.f) ? di[ch] / (float)n : 0.f;
+                }
+                if (ad[0] != 0.f || ad[1] != 0.f || ad[2] != 0.f) {
+                    const DevTex* gt = gradTex >= 0 ? &s.texs[gradTex] : nullptr;
+                    walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
+                        const VertexOut& vo = *pv.vo;
+                        const HitData& hd = *pv.hd;
+                        v3 dfs[6], dfh[6], dfl[6];
+                        disneyEvalGrad(pv.d, hd, vo.bdir, dfs);
+                        disneyEvalGrad(pv.d, hd, vo.hdir, dfh);
+                        if (vo.hasP)
+                            disneyEvalGrad(pv.d, hd, vo.ldir, dfl);
+                        else
+#pragma unroll
+                            for (int q = 0; q < 6; q++) dfl[q] = mk(0, 0, 0);
+                        const v3 La = Lt - pv.Lincl;
+                        const float cosS = fabsf(dot(vo.bdir, hd.normal));
+                        const v3 cS = pv.T * hd.emission * (vo.w3 * cosS / vo.bpdf);
+                        const v3 cH = pv.visH ? pv.T * vo.hval * (vo.w1 * fabsf(dot(vo.hdir, hd.normal)) / vo.hpdf)
+                                              : mk(0, 0, 0);
+                        const v3 cP = (vo.hasP && pv.visP)
+                                          ? pv.T * vo.lrad * (vo.w2 * fabsf(dot(vo.ldir, hd.normal)) / vo.ppdf)
+                                          : mk(0, 0, 0);
+                        const v3 ratio = mk(vo.f.x != 0.f ? La.x / vo.f.x : 0.f, vo.f.y != 0.f ? La.y / vo.f.y : 0.f,
+                                            vo.f.z != 0.f ? La.z / vo.f.z : 0.f);
+                        const v3 kS = (cS + ratio) * mk(ad[0], ad[1], ad[2]);
+                        const v3 kH = cH * mk(ad[0], ad[1], ad[2]);
+                        const v3 kP = cP * mk(ad[0], ad[1], ad[2]);
+                        float g[6];
+#pragma unroll
+                        for (int q = 0; q < 6; q++) g[q] = dot(kS, dfs[q]) + dot(kH, dfh[q]) + dot(kP, dfl[q]);
+                        const int mat = pv.sh->material;
+                        const DevMaterial& m = s.mats[mat];
+                        if (m.albedoTex >= 0) {
+                            if (gt && m.albedoTex == gradTex) {
+                                const float tu = pv.sh->tu, tv = pv.sh->tv;
+                                float* D = dtex;
+                                if (gt->filter == 1) {
+                                    const float xx = tu * gt->w, yy = tv * gt->h;
+                                    const float t1x = floorf(xx), t1y = floorf(yy);
+                                    const float fx = xx - t1x, fy = yy - t1y;
+                                    const int cx[4] = {(int)t1x, (int)t1x + 1, (int)t1x, (int)t1x + 1};
+                                    const int cy[4] = {(int)t1y, (int)t1y, (int)t1y + 1, (int)t1y + 1};
+                                    const float wg[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
+                                    for (int k4 = 0; k4 < 4; k4++) {
+                                        const uint64_t ti = 3 * (uint64_t)texelIndex(*gt, cx[k4], cy[k4]);
+                                        for (int q = 0; q < 3; q++) atomicAdd(D + ti + q, g[q] * wg[k4]);
+                                    }
+                                } else {
+                                    const uint64_t ti = 3 * (uint64_t)texelIndex(*gt, (int)(tu * gt->w), (int)(tv * gt->h));
+                                    for (int q = 0; q < 3; q++) atomicAdd(D + ti + q, g[q]);
+                                }
+                            }
+                            g[0] = g[1] = g[2] = 0.f;
+                        }
+                        const float e = 2.2f;
+                        g[3] = m.roughnessTex < 0 ? g[3] * e * powf(m.roughness, e - 1.0f) : 0.f;
+                        g[4] = m.metallicTex < 0 ? g[4] * e * powf(m.metallic, e - 1.0f) : 0.f;
+                        if (nrec < kAdjMaxDepth) {
+                            recMat[nrec] = mat;
+#pragma unroll
+                            for (int q = 0; q < 6; q++) recG[nrec][q] = g[q];
+                            nrec++;
+                        }
+                    });
+                }
+            }
+        }
+        overflow |= st.overflow;
+        // converged: reduce the records per material across the wave
+        for (int r = 0; r < (int)a.maxDepth && r < kAdjMaxDepth; r++) {
+            const int m = r < nrec ? recMat[r] : -1;
+            unsigned long long pending = __ballot(m >= 0);
+            while (pending) {
+                const int leader = __ffsll((long long)pending) - 1;
+                const int m0 = __shfl(m, leader);
+                const bool mine = m == m0;
+                const unsigned long long grp = __ballot(mine);
+#pragma unroll
+                for (int q = 0; q < 6; q++) {
+                    float v = mine ? recG[r][q] : 0.f;
+#pragma unroll
+                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+                    if ((int)lane == leader && v != 0.f) atomicAdd(dmat + (uint64_t)m0 * 6 + q, (double)v);
+                }
+                pending &= ~grp;
             }
         }
     }
@@ -628,6 +828,21 @@ hipError_t launchPlain(const LaunchCfg& c, const DevScene& s, const RenderArgs& 
     else
         hipLaunchKernelGGL(k_plain<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, accum, counts, aov, c.gstack,
                            c.maxThreads, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchAdjoint(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, uint32_t spp,
+                         const float* dimage, const uint32_t* counts, int gradTex, double* dmat, float* dtex,
+                         unsigned long long* counters) {
+    uint64_t total = (uint64_t)((s.cam.xRes + 15) / 16) * ((s.cam.yRes + 15) / 16) * 256 * spp;
+    uint32_t grid = gridFor(total, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+    if (s.width == 4)
+        hipLaunchKernelGGL(k_adjoint<4>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, gradTex,
+                           dmat, dtex, c.gstack, c.maxThreads, counters);
+    else
+        hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, gradTex,
+                           dmat, dtex, c.gstack, c.maxThreads, counters);
     return hipGetLastError();
 }
 

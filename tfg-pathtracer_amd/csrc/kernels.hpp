@@ -21,6 +21,10 @@ constexpr int kMaxBlock = 256;
 constexpr uint32_t kDefaultBlock = 64;
 constexpr uint32_t kDefaultRefill = 16;
 
+namespace dev {
+constexpr int kAdjMaxDepthHost = 16;   // deepest path the adjoint records (k_adjoint's kAdjMaxDepth)
+}
+
 // Tally slots (per XCD copy): the host sums the kXcds copies.
 constexpr int kStatSlots = 8;
 constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5;
@@ -69,6 +73,9 @@ hipError_t launchTrace(const LaunchCfg& c, const dev::DevScene& s, const float* 
                        unsigned long long* counters);
 hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, float4* accum,
                        uint32_t* counts, float4* aov, unsigned long long* counters);
+hipError_t launchAdjoint(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, uint32_t spp,
+                         const float* dimage, const uint32_t* counts, int gradTex, double* dmat, float* dtex,
+                         unsigned long long* counters);
 hipError_t launchWfCamera(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q);
 hipError_t launchWfExtend(const LaunchCfg& c, const dev::DevScene& s, const WfState& q, const float4* rayIn,
                           const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount, bool countTraversal,

@@ -153,6 +153,14 @@ __device__ __forceinline__ v3 texel(const float* __restrict__ pool, const DevTex
     const float* p = pool + t.offset + 3 * ((uint64_t)y * t.w + x);
     return ld3(p);
 }
+// Index (texel units, within the texture) that texel() reads for (x, y).
+__device__ __forceinline__ uint32_t texelIndex(const DevTex& t, int x, int y) {
+    x = (int)(t.xTile * (x + t.xOffset * t.w)) % t.w;
+    y = (int)(t.yTile * (y + t.yOffset * t.h)) % t.h;
+    if (x < 0) x += t.w;
+    if (y < 0) y += t.h;
+    return (uint32_t)y * (uint32_t)t.w + (uint32_t)x;
+}
 __device__ __forceinline__ v3 texUV(const float* pool, const DevTex& t, float u, float v) {
     return texel(pool, t, (int)(u * t.w), (int)(v * t.h));
 }
@@ -321,6 +329,78 @@ __device__ __forceinline__ v3 disneyEval(v3 dir, const HitData& h, v3 L) {
                Gs * Fs * Ds + mk(1, 1, 1) * (0.25f * h.clearcoat * Gr * Fr * Dr);
     }
     return brdf;
+}
+
+// d disneyEval / d (albedo.x, albedo.y, albedo.z, roughness, metallic,
+// specular) of HitData (effective values, after generateHitData's pow 2.2),
+// written out analytically from the expression above; used by the adjoint.
+__device__ __forceinline__ void disneyEvalGrad(v3 dir, const HitData& h, v3 L, v3 (&df)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) df[k] = mk(0, 0, 0);
+    v3 V = dir * -1.0f;
+    v3 T, B;
+    createBasis(h.normal, T, B);
+    v3 H = normalized(L + V);
+    float NDotL = fabsf(dot(h.normal, L));
+    float NDotV = fabsf(dot(h.normal, V));
+    float NDotH = fabsf(dot(h.normal, H));
+    float LDotH = fabsf(dot(L, H));
+    if (!(h.transmission < 1.0f && dot(h.normal, L) > 0.0f && dot(h.normal, V) > 0.0f)) return;
+    const v3 C = h.albedo;
+    const float wl[3] = {0.3f, 0.6f, 0.1f};
+    const float lum = wl[0] * C.x + wl[1] * C.y + wl[2] * C.z;
+    const v3 Ctint = lum > 0.0f ? C / lum : mk(1, 1, 1);
+    const float mu = h.metallic, rho = h.roughness;
+    const v3 tintMix = mk(1, 1, 1) + (Ctint - mk(1, 1, 1)) * h.specularTint;
+    const float S0 = h.specular * 0.08f;
+    const v3 Cspec0 = S0 * tintMix + (C - S0 * tintMix) * mu;
+    const v3 Csheen = mk(1, 1, 1) + (Ctint - mk(1, 1, 1)) * h.sheenTint;
+    const float FL = schlickFresnel(NDotL), FV = schlickFresnel(NDotV), FH = schlickFresnel(LDotH);
+    const float Fd90 = 0.5f + 2.0f * LDotH * LDotH * rho;
+    const float A = 1 + FL * (Fd90 - 1), Bd = 1 + FV * (Fd90 - 1);
+    const float Fd = A * Bd, dFd = 2 * LDotH * LDotH * (FL * Bd + FV * A);
+    const float Fss90 = LDotH * LDotH * rho;
+    const float A2 = 1 + FL * (Fss90 - 1), B2 = 1 + FV * (Fss90 - 1);
+    const float Fss = A2 * B2, dFss = LDotH * LDotH * (FL * B2 + FV * A2);
+    const float k1 = 1.0f / (NDotL + NDotV) - 0.5f;
+    const float ss = 1.25f * (Fss * k1 + 0.5f), dss = 1.25f * k1 * dFss;
+    const float D0 = Fd + h.subsurface * (ss - Fd), dD0 = dFd + h.subsurface * (dss - dFd);
+    const float aspect = sqrtf(1.0f - h.anisotropic * 0.9f);
+    const float ax = maxf(0.001f, rho / aspect), ay = maxf(0.001f, rho * aspect);
+    const float dax = rho / aspect > 0.001f ? 1.0f / aspect : 0.0f, day = rho * aspect > 0.001f ? aspect : 0.0f;
+    const float HX = dot(H, T), HY = dot(H, B);
+    const float a = HX / ax, b = HY / ay, c = a * a + b * b + NDotH * NDotH;
+    const float Ds = 1.0f / (EL_PI * ax * ay * c * c);
+    const float dDs = Ds * (-1.0f / ax + 4.0f * a * a / (c * ax)) * dax + Ds * (-1.0f / ay + 4.0f * b * b / (c * ay)) * day;
+    float GL, dGL, GV, dGV;
+    {
+        float X = dot(L, T), Y = dot(L, B);
+        float q = sqrtf((X * ax) * (X * ax) + (Y * ay) * (Y * ay) + NDotL * NDotL);
+        GL = 1.0f / (NDotL + q);
+        dGL = -GL * GL * ((X * X * ax) * dax + (Y * Y * ay) * day) / q;
+    }
+    {
+        float X = dot(V, T), Y = dot(V, B);
+        float q = sqrtf((X * ax) * (X * ax) + (Y * ay) * (Y * ay) + NDotV * NDotV);
+        GV = 1.0f / (NDotV + q);
+        dGV = -GV * GV * ((X * X * ax) * dax + (Y * Y * ay) * day) / q;
+    }
+    const float Gs = GL * GV, dGs = dGL * GV + GL * dGV;
+    const v3 Fs = Cspec0 + (mk(1, 1, 1) - Cspec0) * FH;
+    const v3 Fsheen = Csheen * (FH * h.sheen);
+    const v3 diffusePart = C * ((1.0f / EL_PI) * D0) + Fsheen;
+    const float spec = Gs * Ds * (1.0f - FH);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const v3 e = mk(k == 0 ? 1.f : 0.f, k == 1 ? 1.f : 0.f, k == 2 ? 1.f : 0.f);
+        const v3 dCtint = lum > 0.0f ? e / lum - C * (wl[k] / (lum * lum)) : mk(0, 0, 0);
+        const v3 dCspec0 = (S0 * h.specularTint * (1.0f - mu)) * dCtint + e * mu;
+        const v3 dFsheen = dCtint * (FH * h.sheen * h.sheenTint);
+        df[k] = (e * ((1.0f / EL_PI) * D0) + dFsheen) * (1.0f - mu) + dCspec0 * spec;
+    }
+    df[3] = C * ((1.0f / EL_PI) * dD0 * (1.0f - mu)) + Fs * (dGs * Ds + Gs * dDs);
+    df[4] = diffusePart * -1.0f + (C - S0 * tintMix) * spec;
+    df[5] = tintMix * (0.08f * (1.0f - mu) * spec);
 }
 
 // ----------------------------------------------------------------- camera --

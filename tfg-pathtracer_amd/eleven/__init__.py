@@ -86,7 +86,12 @@ class el_stats(C.Structure):
                 ("shade_ms", C.c_double), ("extend_launches", C.c_uint64), ("extend_rays", C.c_uint64),
                 ("node_fetches", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64), ("bvh_width", C.c_uint32),
-                ("stack_bound", C.c_uint32), ("node_stride", C.c_uint32), ("tri_stride", C.c_uint32)]
+                ("stack_bound", C.c_uint32), ("node_stride", C.c_uint32), ("tri_stride", C.c_uint32),
+                ("adjoint_ms", C.c_double), ("adjoint_rays", C.c_uint64)]
+
+
+class el_adjoint_params(C.Structure):
+    _fields_ = [("grad_texture", C.c_int32), ("block", C.c_uint32), ("flags", C.c_uint32)]
 
 
 # Every symbol declared in include/eleven.h, with its ctypes signature.
@@ -110,6 +115,7 @@ _SIGS = {
     "el_get_stats": (C.c_int, [_P, C.POINTER(el_stats)]),
     "el_set_materials": (C.c_int, [_P, C.POINTER(el_material), C.c_uint32]),
     "el_set_texture": (C.c_int, [_P, C.c_int32, _P]),
+    "el_adjoint": (C.c_int, [_P, C.POINTER(el_adjoint_params), _P, _P, _P, C.c_int]),
     "el_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, C.c_int]),
     "el_trace_any": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_int]),
     "el_trace_count": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int]),
@@ -363,6 +369,25 @@ class Context:
     def set_materials(self, mats):
         arr = (el_material * len(mats))(*mats)
         self._check(lib().el_set_materials(self._h, arr, len(mats)))
+
+    def adjoint(self, dimage, grad_texture=-1, block=0):
+        """Path-replay adjoint of the accumulated image: gradients of
+        sum(dimage * I) (dimage [H][W][3], the layout of read()) with respect to
+        every material's (albedo r, g, b, roughness, metallic, specular) and the
+        texels of texture `grad_texture`.  Returns (dmat [M][6], dtex [h][w][3]
+        or None)."""
+        cam = self.scene.camera
+        di = np.ascontiguousarray(dimage, np.float32).reshape(cam.y_res, cam.x_res, 3)
+        m = len(self.scene.materials())
+        dmat = np.zeros((m, 6), np.float32)
+        dtex = None
+        if grad_texture >= 0:
+            t = self.scene.view.textures[grad_texture]
+            dtex = np.zeros((t.height, t.width, 3), np.float32)
+        p = el_adjoint_params(grad_texture, block, 0)
+        self._check(lib().el_adjoint(self._h, C.byref(p), di.ctypes.data, dmat.ctypes.data,
+                                     dtex.ctypes.data if dtex is not None else None, 0))
+        return dmat, dtex
 
     def set_texture(self, tex_id, rgb):
         a = np.ascontiguousarray(rgb, np.float32)

@@ -1,0 +1,85 @@
+"""torch.autograd binding of the renderer.
+
+    image = Renderer(ctx, scene, spp, depth, seed)(params, texture)
+
+renders the scene with per-material parameters `params` [M][6] (albedo r, g,
+b, roughness, metallic, specular: the values stored in el_material) and,
+optionally, the texels of one texture, and returns the mean image [H][W][3]
+as a float32 CPU tensor.  Its backward runs the library's path-replay
+adjoint (el_adjoint) on the accumulation the forward left in the context, so
+a backward must follow its forward before the context renders again.
+
+The renderer is deterministic for a fixed seed (Philox counters per pixel,
+sample and bounce), so forward calls with the same seed use common random
+numbers: finite differences of the loss are differences of the same paths.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import MODE_WAVEFRONT, Context, SceneData, el_material
+
+PARAM_NAMES = ("albedo_r", "albedo_g", "albedo_b", "roughness", "metallic", "specular")
+
+
+def material_params(scene: SceneData) -> np.ndarray:
+    """[M][6] float32: each material's (albedo rgb, roughness, metallic, specular)."""
+    out = []
+    for m in scene.materials():
+        out.append([m.albedo.x, m.albedo.y, m.albedo.z, m.roughness, m.metallic, m.specular])
+    return np.asarray(out, np.float32)
+
+
+def with_params(base: list, params: np.ndarray) -> list:
+    """Copies of `base` el_material structs with the six parameters replaced."""
+    out = []
+    for m, p in zip(base, np.asarray(params, np.float64)):
+        c = el_material()
+        C.pointer(c)[0] = m
+        c.albedo.x, c.albedo.y, c.albedo.z = float(p[0]), float(p[1]), float(p[2])
+        c.roughness, c.metallic, c.specular = float(p[3]), float(p[4]), float(p[5])
+        out.append(c)
+    return out
+
+
+class Renderer:
+    """Differentiable forward render of one uploaded scene."""
+
+    def __init__(self, ctx: Context, scene: SceneData, spp: int, depth: int = 5, seed: int = 0,
+                 grad_texture: int = -1, mode: int = MODE_WAVEFRONT):
+        self.ctx, self.scene = ctx, scene
+        self.spp, self.depth, self.seed = spp, depth, seed
+        self.grad_texture, self.mode = grad_texture, mode
+        self.base = scene.materials()
+
+    def render(self, params: np.ndarray, texture: np.ndarray | None = None) -> np.ndarray:
+        """Forward only (numpy in, numpy out)."""
+        self.ctx.set_materials(with_params(self.base, params))
+        if texture is not None:
+            self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
+        self.ctx.clear()
+        self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode)
+        return self.ctx.read()[0][..., :3].copy()
+
+    def __call__(self, params: torch.Tensor, texture: torch.Tensor | None = None) -> torch.Tensor:
+        return _RenderFn.apply(params, texture, self)
+
+
+class _RenderFn(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, params, texture, r: Renderer):
+        tex = texture.detach().cpu().numpy() if texture is not None else None
+        img = r.render(params.detach().cpu().numpy(), tex)
+        fctx.r = r
+        fctx.has_tex = texture is not None
+        return torch.from_numpy(img)
+
+    @staticmethod
+    def backward(fctx, grad_img):
+        r = fctx.r
+        gt = r.grad_texture if fctx.has_tex else -1
+        dmat, dtex = r.ctx.adjoint(grad_img.detach().cpu().numpy(), grad_texture=gt)
+        return torch.from_numpy(dmat), (torch.from_numpy(dtex) if fctx.has_tex else None), None
