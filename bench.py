@@ -12,9 +12,12 @@ extension + HDRI shadow + point-light shadow), the same logical count for
 the plain and wavefront renderers.
 
 Extra fields: `plain` (the one-thread-per-pixel megakernel on the same
-frame), `roofline` of the dominant kernel (closest-hit extension), and
+frame), `roofline` of the dominant kernel (closest-hit extension),
 `cpu_baseline` (the CPU oracle, a restatement of the reference estimator,
-on a bounded crop of the same frame using all host cores).
+on a bounded crop of the same frame using all host cores) and `adjoint`
+(the metric's second part: gradients per second of forward + path-replay
+adjoint on the 64-sphere gradient-check scene, with the FP64 CPU adjoint
+timed beside it).
 """
 from __future__ import annotations
 
@@ -84,6 +87,62 @@ def cpu_baseline(scene, budget_s=15.0, threads=None):
                       f"14-bin SAH binary BVH; tree build {build_s:.1f} s not timed)"}
 
 
+def adjoint_bench(steps=5, with_cpu=True):
+    """Gradients per second of the inverse-rendering step on the gradient-check
+    scene: 64-sphere field (61,442 tris) at 128x128, 32 spp, depth 4; the
+    gradient covers 65 materials x 6 parameters + a 256x256 RGB albedo texture
+    (196,608 texels).  One gradient = forward render (wavefront) + adjoint
+    (path replay) of an L2 loss against a target rendered with perturbed
+    parameters; parameter and texture uploads included."""
+    import eleven
+    from eleven import scenegen
+    from eleven.autograd import Renderer, material_params
+    import numpy as np
+    d = scenegen.cached("sphere_field", n_spheres=64, res=(128, 128), seed=0, ground_tex_size=256)
+    scene = eleven.SceneData.load(d)
+    ctx = eleven.Context(0)
+    ctx.upload(scene)
+    r = Renderer(ctx, scene, spp=32, depth=4, seed=1, grad_texture=0)
+    p0 = material_params(scene)
+    t0 = scene.texture(0).copy()
+    rng = np.random.default_rng(0)
+    pt = np.clip(p0 + rng.normal(0, 0.1, p0.shape).astype(np.float32), 0.0, 1.0)
+    pt[:, 3] = np.maximum(pt[:, 3], 0.05)
+    target = r.render(pt, t0)
+    img = r.render(p0, t0)
+    ctx.adjoint(2 * (img - target), grad_texture=0)      # warm-up
+    fwd_ms = adj_ms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        img = r.render(p0, t0)
+        fwd_ms += ctx.stats().render_ms
+        dmat, dtex = ctx.adjoint(2 * (img - target), grad_texture=0)
+        adj_ms += ctx.stats().adjoint_ms
+    wall = time.perf_counter() - t_start
+    out = {"metric": "adjoint grads/s", "value": round(steps / wall, 3), "unit": "grads/s",
+           "config": "sphere field 61,442 tris, 128x128, 32 spp, depth 4, gradient of 65 materials x 6 "
+                     "parameters + 256x256 RGB albedo texture (L2 loss)",
+           "ms_per_grad": round(wall / steps * 1e3, 2), "forward_ms": round(fwd_ms / steps, 2),
+           "adjoint_ms": round(adj_ms / steps, 2), "grad_norm": float(np.linalg.norm(dmat))}
+    ctx.close()
+    if with_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_py as orc
+        threads = min(16, os.cpu_count() or 1)
+        rb = orc.RefBVH(scene.vertices())
+        w = (2 * (img - target)).astype(np.float32)
+        t0c = time.perf_counter()
+        n = 0
+        while n < 50 and time.perf_counter() - t0c < 8.0:
+            orc.adjoint(scene, rb, w, 32, depth=4, seed=1, precision=1, grad_texture=0, threads=threads)
+            n += 1
+        dt = time.perf_counter() - t0c
+        out["cpu_baseline"] = {"value": round(n / dt, 4), "unit": "grads/s", "cores": threads, "kind": "port",
+                               "sample": f"FP64 oracle adjoint (primal walk + replay) of the same loss and "
+                                         f"configuration, {n} gradients in {dt:.2f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +160,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--save", default="")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches): the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings")
@@ -212,6 +272,9 @@ def main():
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene)
     ctx.close()
+    if not args.no_adjoint:
+        log("adjoint ...")
+        out["adjoint"] = adjoint_bench(with_cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

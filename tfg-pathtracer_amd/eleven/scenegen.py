@@ -177,9 +177,24 @@ def uv_sphere(center, radius, seg=32, rings=16):
     return np.array(verts), np.array(uvs), np.array(nrms), np.array(faces)
 
 
+def checker_noise_texture(size, seed):
+    """size x size 8-bit RGB: an 8x8 checker of two seeded colours modulated by value noise."""
+    rng = np.random.default_rng(seed)
+    c0, c1 = rng.uniform(0.2, 0.9, 3), rng.uniform(0.2, 0.9, 3)
+    y, x = np.mgrid[0:size, 0:size] / size
+    chk = ((np.floor(x * 8) + np.floor(y * 8)) % 2)[..., None]
+    n = value_noise(x * 8, y * 8, seed, octaves=4)[..., None]
+    rgb = (c0 * (1 - chk) + c1 * chk) * (0.75 + 0.25 * n)
+    return (np.clip(rgb, 0, 1) * 255).astype(np.uint8)
+
+
 def sphere_field(out_dir, n_spheres=64, res=(256, 256), seed=0, seg=32, rings=16, ground_texture=None,
-                 env=0.5, lights=2, light_radiance=50.0, extent=10.0, hdri_sky=None):
-    """Sphere-field scene (the 61,442-triangle workload at n_spheres=64)."""
+                 env=0.5, lights=2, light_radiance=50.0, extent=10.0, hdri_sky=None, ground_tex_size=0):
+    """Sphere-field scene (the 61,442-triangle workload at n_spheres=64).
+    ground_tex_size > 0: the ground gets a seeded checker/noise albedo map of
+    that size (ground_texture gives one explicitly)."""
+    if ground_texture is None and ground_tex_size > 0:
+        ground_texture = checker_noise_texture(ground_tex_size, seed + 1000)
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     rng = np.random.default_rng(seed)
