@@ -119,6 +119,8 @@ typedef struct {
     uint32_t flags;        /* EL_FLAG_*                                     */
     uint32_t refill;       /* wavefront traversal: idle lanes (1..64) that
                               trigger an in-wave ray refill, 0 -> default   */
+    uint32_t rng_scene;    /* scene word of the RNG counter (a batch uses
+                              the scene's index there), 0 by default        */
 } el_render_params;
 
 typedef struct {
@@ -240,6 +242,46 @@ int  el_device_alloc(el_ctx* ctx, size_t bytes, void** out);
 int  el_device_free(el_ctx* ctx, void* ptr);
 int  el_memcpy_htod(el_ctx* ctx, void* dst, const void* src, size_t bytes);
 int  el_memcpy_dtoh(el_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* ---- batches: many scenes through one wavefront loop ------------------
+ * No counterpart in the reference (it renders one scene per process).  Each
+ * scene keeps its own spp budget, seed and camera; the batch enumerates the
+ * work of all scenes, fills each generation of queue_cap paths across scene
+ * boundaries and runs the camera / extend / shade / shadow / finalize
+ * sequence over the mixed queues.  Scene i of a batch draws exactly the
+ * random numbers of el_render with seed = its seed and rng_scene = i. */
+typedef struct el_batch el_batch;
+
+#define EL_BATCH_GRAPH 1u   /* capture the launch sequence in a hipGraph once,
+                               replay it on later renders with equal plans  */
+
+typedef struct {
+    uint32_t max_depth;    /* 0 -> 5                                        */
+    uint32_t block;        /* traversal threads per block, 0 -> default     */
+    uint32_t queue_cap;    /* paths in flight, 0 -> a quarter of free HBM   */
+    uint32_t bvh_width;    /* 4 or 8, 0 -> current                          */
+    uint32_t refill;       /* in-wave refill threshold, 0 -> default        */
+    uint32_t flags;        /* EL_BATCH_*                                    */
+} el_batch_params;
+
+int  el_batch_create(el_batch** out, int device);
+void el_batch_destroy(el_batch* batch);
+const char* el_batch_last_error(const el_batch* batch);
+/* Trials and choice of the last el_batch_tune, one line per trial. */
+const char* el_batch_log(const el_batch* batch);
+/* Uploads a scene (the library copies it) with its spp budget and seed;
+ * *index receives its position in the batch. */
+int  el_batch_add(el_batch* batch, const el_scene_view* view, uint32_t spp,
+                  uint64_t seed, int32_t* index);
+/* Renders every scene's spp from cleared accumulations (synchronous). */
+int  el_batch_render(el_batch* batch, const el_batch_params* params);
+/* Autotuner: coordinate descent over bvh_width, queue_cap, block and refill,
+ * each trial rendering every scene at min(spp, trial_spp) samples; writes
+ * the fastest setting into *params (flags are kept). */
+int  el_batch_tune(el_batch* batch, el_batch_params* params, uint32_t trial_spp);
+/* Mean RGBA [H][W][4] and accepted-sample counts of scene `scene`. */
+int  el_batch_read(el_batch* batch, int32_t scene, float* rgba, uint32_t* samples);
+int  el_batch_stats(el_batch* batch, el_stats* out);
 
 #ifdef __cplusplus
 }

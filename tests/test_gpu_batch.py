@@ -1,0 +1,119 @@
+"""Batched rendering (el_batch_*): many scenes through one wavefront loop.
+
+1. Scene i of a batch draws exactly the random numbers of el_render with its
+   seed and rng_scene = i, so its image equals a standalone wavefront render
+   of that scene: accepted-sample counts equal, radiance within rtol 1e-4
+   (FP32 contraction differences between the one-scene and many-scene
+   instantiations of the shading code, as for plain vs wavefront) -- with
+   one generation for the whole batch and with a small queue whose
+   generations cut across scene boundaries.
+2. The hipGraph-captured launch sequence, replayed several times with eager
+   renders and synchronisations between the replays, reproduces the eager
+   result bit for bit every time.
+3. The autotuner returns a valid setting, logs its trials, and the batch
+   rendered with that setting equals the default one (traversal results and
+   per-pixel accumulation order do not depend on the launch shape).
+"""
+import numpy as np
+import pytest
+
+import eleven
+from eleven import scenegen
+
+DEPTH = 4
+SPECS = [((40, 24), 3, 2, 11), ((64, 64), 5, 12, 12), ((17, 33), 2, 1, 13), ((48, 48), 4, 6, 14)]
+
+
+@pytest.fixture(scope="module")
+def batch_scenes():
+    out = []
+    for (res, spp, nsph, seed) in SPECS:
+        sky = scenegen.sky_hdr(64, 32, sun_dir=(0.3, 0.7, 0.2), peak=30.0, sigma_deg=8.0)
+        out.append((scenegen.sphere_field_arrays(n_spheres=nsph, res=res, seed=seed, hdri=sky), spp, seed))
+    return out
+
+
+def _standalone(ctx, scene, spp, seed, rng_scene):
+    ctx.upload(scene)
+    ctx.render(spp, depth=DEPTH, seed=seed, mode=eleven.MODE_WAVEFRONT, rng_scene=rng_scene)
+    img, cnt, _, _ = ctx.read()
+    return img, cnt
+
+
+def _batch(scenes):
+    b = eleven.Batch(0)
+    for s, spp, seed in scenes:
+        b.add(s, spp, seed)
+    return b
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("queue", [0, 2048])
+def test_batch_equals_standalone_renders(gpu_ctx, batch_scenes, queue):
+    print(f"[batch] batch vs standalone, queue {queue}", flush=True)
+    b = _batch(batch_scenes)
+    st = b.render(depth=DEPTH, queue_cap=queue)
+    if queue:
+        assert st.extend_launches > DEPTH   # several generations
+    total_rays = 0
+    for i, (s, spp, seed) in enumerate(batch_scenes):
+        img, cnt = b.read(i)
+        ref, rcnt = _standalone(gpu_ctx, s, spp, seed, i)
+        total_rays += gpu_ctx.stats().rays
+        np.testing.assert_array_equal(cnt, rcnt)
+        np.testing.assert_allclose(img, ref, rtol=1e-4, atol=1e-6)
+        assert cnt.max() == spp
+    assert st.rays == total_rays
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_match_eager(batch_scenes):
+    print("[batch] hipGraph replays vs eager", flush=True)
+    b = _batch(batch_scenes)
+    b.render(depth=DEPTH, queue_cap=4096)
+    eager = [b.read(i) for i in range(len(batch_scenes))]
+    for rep in range(3):
+        b.render(depth=DEPTH, queue_cap=4096, graph=True)
+        for i in range(len(batch_scenes)):
+            img, cnt = b.read(i)
+            np.testing.assert_array_equal(cnt, eager[i][1])
+            np.testing.assert_array_equal(img, eager[i][0])
+        b.render(depth=DEPTH, queue_cap=4096)   # eager work between replays
+        np.testing.assert_array_equal(b.read(0)[0], eager[0][0])
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_autotuner(batch_scenes):
+    print("[batch] autotuner", flush=True)
+    b = _batch(batch_scenes)
+    b.render(depth=DEPTH)
+    ref = [b.read(i) for i in range(len(batch_scenes))]
+    cfg = b.tune(depth=DEPTH, trial_spp=1)
+    log = b.log()
+    print(log, flush=True)
+    assert "chosen" in log and log.count(" ms") >= 5
+    assert cfg["bvh_width"] in (0, 4, 8) and cfg["block"] in (0, 64, 128, 256)
+    b.render(depth=DEPTH, **cfg)
+    for i in range(len(batch_scenes)):
+        img, cnt = b.read(i)
+        np.testing.assert_array_equal(cnt, ref[i][1])
+        np.testing.assert_allclose(img, ref[i][0], rtol=1e-6, atol=1e-8)
+    b.close()
+
+
+@pytest.mark.gpu
+def test_batch_errors():
+    b = eleven.Batch(0)
+    with pytest.raises(eleven.ElevenError):
+        b.render(depth=DEPTH)                 # empty batch
+    s = scenegen.sphere_field_arrays(n_spheres=1, res=(8, 8), seed=1)
+    with pytest.raises(eleven.ElevenError):
+        b.add(s, 0, 1)                        # zero spp
+    b.add(s, 1, 1)
+    with pytest.raises(eleven.ElevenError):
+        b.render(depth=DEPTH, bvh_width=6)
+    with pytest.raises(eleven.ElevenError):
+        b.read(5)
+    b.close()

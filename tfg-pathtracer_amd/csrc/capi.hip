@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -96,8 +97,8 @@ struct el_ctx {
     DevBuf tRays, tTmax, tTri, tT, tUV, tOcc;   // scratch for host-pointer ray queries
 
     uint32_t sampleBase = 0;
-    uint64_t accSeed = 0;          // seed / depth of the renders in the accumulation
-    uint32_t accDepth = 0;
+    uint64_t accSeed = 0;          // seed / depth / RNG scene word of the renders in the accumulation
+    uint32_t accDepth = 0, accScene = 0;
     bool accMixed = false;         // renders since el_clear used different seeds or depths
     uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
     DevBuf aImage, aMat, aTex;     // adjoint image, FP64 material and FP32 texel gradients
@@ -459,7 +460,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     RenderArgs a{};
     a.k0 = (uint32_t)p->seed;
     a.k1 = (uint32_t)(p->seed >> 32);
-    a.scene = 0;
+    a.scene = p->rng_scene;
     a.maxDepth = depth;
     a.sample = c->sampleBase;
 
@@ -573,7 +574,8 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     if (c->sampleBase == 0) {
         c->accSeed = p->seed;
         c->accDepth = depth;
-    } else if (c->accSeed != p->seed || c->accDepth != depth) {
+        c->accScene = p->rng_scene;
+    } else if (c->accSeed != p->seed || c->accDepth != depth || c->accScene != p->rng_scene) {
         c->accMixed = true;
     }
     c->sampleBase += p->spp;
@@ -627,7 +629,7 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     RenderArgs a{};
     a.k0 = (uint32_t)c->accSeed;
     a.k1 = (uint32_t)(c->accSeed >> 32);
-    a.scene = 0;
+    a.scene = c->accScene;
     a.maxDepth = c->accDepth;
     a.sample = 0;
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
@@ -888,3 +890,416 @@ int el_memcpy_dtoh(el_ctx* c, void* dst, const void* src, size_t bytes) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- batches --
+// Many scenes rendered by one wavefront loop.  Each scene is uploaded into
+// its own context (geometry, textures, HDRI tables and its accumulation stay
+// in that context's buffers); the batch owns the queues, the traversal
+// stack, a device array of the scenes' DevScene records and the generation
+// plan.  Work items are enumerated scene after scene, sample-major inside a
+// scene; a generation takes the next queue_cap of them, whatever scenes they
+// belong to, and a segment table tells the camera and finalize kernels which
+// scene each path slot belongs to.  With EL_BATCH_GRAPH the whole launch
+// sequence is captured once into a hipGraph and replayed.
+struct el_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err, log;
+    uint32_t maxThreads = 0;
+    std::vector<el_ctx*> scenes;
+    std::vector<uint32_t> spp;
+    std::vector<uint64_t> seed;
+    int width = 4;
+    DevBuf dScenes, dSegs, dGstack, dCounters;
+    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wCounters;
+    size_t wfCap = 0, wfQcap = 0;
+    hipGraphExec_t graph = nullptr;
+    std::string graphKey;
+    el_stats stats{};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+static int bfail(el_batch* b, int code, const std::string& what) {
+    if (b) b->err = what;
+    return code;
+}
+
+#define EL_BHIP(b, x)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return bfail(b, EL_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct BatchGen {
+    size_t segOffset;
+    uint32_t nseg, count, finTotal;
+};
+
+extern "C" {
+
+int el_batch_create(el_batch** out, int device) {
+    if (!out) return EL_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return EL_ERR_HIP;
+    if (device < 0 || device >= n) return EL_ERR_ARG;
+    std::unique_ptr<el_batch> b(new el_batch());
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess) return EL_ERR_HIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return EL_ERR_HIP;
+    b->maxThreads = (uint32_t)prop.multiProcessorCount * 2048u;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
+    if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) return EL_ERR_HIP;
+    if (b->dCounters.ensure(sizeof(unsigned long long) * kCounterWords) != hipSuccess) return EL_ERR_HIP;
+    *out = b.release();
+    return EL_OK;
+}
+
+void el_batch_destroy(el_batch* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto* c : b->scenes) el_destroy(c);
+    if (b->e0) (void)hipEventDestroy(b->e0);
+    if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+const char* el_batch_last_error(const el_batch* b) { return b ? b->err.c_str() : "null batch"; }
+const char* el_batch_log(const el_batch* b) { return b ? b->log.c_str() : ""; }
+
+int el_batch_add(el_batch* b, const el_scene_view* view, uint32_t spp, uint64_t seed, int32_t* index) {
+    if (!b || !view) return bfail(b, EL_ERR_ARG, "null argument");
+    if (spp == 0) return bfail(b, EL_ERR_ARG, "spp must be positive");
+    if (b->scenes.size() >= 65536) return bfail(b, EL_ERR_RANGE, "at most 65536 scenes per batch");
+    el_ctx* c = nullptr;
+    int rc = el_create(&c, b->device);
+    if (rc) return bfail(b, rc, "context creation failed");
+    rc = el_upload_scene(c, view);
+    if (rc) {
+        std::string m = el_last_error(c);
+        el_destroy(c);
+        return bfail(b, rc, m);
+    }
+    if (c->width != b->width) {
+        rc = uploadBvh(c, b->width);
+        if (rc) {
+            std::string m = el_last_error(c);
+            el_destroy(c);
+            return bfail(b, rc, m);
+        }
+    }
+    b->scenes.push_back(c);
+    b->spp.push_back(spp);
+    b->seed.push_back(seed);
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    if (index) *index = (int32_t)b->scenes.size() - 1;
+    return EL_OK;
+}
+
+int el_batch_read(el_batch* b, int32_t scene, float* rgba, uint32_t* samples) {
+    if (!b) return EL_ERR_ARG;
+    if (scene < 0 || (size_t)scene >= b->scenes.size()) return bfail(b, EL_ERR_ARG, "scene index out of range");
+    int rc = el_read_passes(b->scenes[scene], rgba, samples, nullptr, nullptr);
+    if (rc) return bfail(b, rc, el_last_error(b->scenes[scene]));
+    return EL_OK;
+}
+
+int el_batch_stats(el_batch* b, el_stats* out) {
+    if (!b || !out) return EL_ERR_ARG;
+    *out = b->stats;
+    return EL_OK;
+}
+
+}  // extern "C"
+
+static int batchEnsureWf(el_batch* b, size_t cap, size_t qcap) {
+    if (cap <= b->wfCap && qcap <= b->wfQcap) return EL_OK;
+    EL_BHIP(b, b->wRay0.ensure(qcap * 32));
+    EL_BHIP(b, b->wRay1.ensure(qcap * 32));
+    EL_BHIP(b, b->wHits.ensure(qcap * 16));
+    EL_BHIP(b, b->wShH.ensure(qcap * 48));
+    EL_BHIP(b, b->wShP.ensure(qcap * 48));
+    EL_BHIP(b, b->wT.ensure(cap * 16));
+    EL_BHIP(b, b->wL.ensure(cap * 16));
+    EL_BHIP(b, b->wPix.ensure(cap * 4));
+    EL_BHIP(b, b->wSample.ensure(cap * 4));
+    EL_BHIP(b, b->wCounters.ensure(kQCounters * sizeof(uint32_t)));
+    b->wfCap = std::max(b->wfCap, cap);
+    b->wfQcap = std::max(b->wfQcap, qcap);
+    return EL_OK;
+}
+
+// Renders every scene of the batch from a cleared accumulation with spp[i]
+// (capped at sppCap if nonzero).  Returns the device time in *ms.
+static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, double* ms) {
+    const size_t S = b->scenes.size();
+    if (S == 0) return bfail(b, EL_ERR_STATE, "empty batch");
+    const uint32_t depth = p->max_depth ? p->max_depth : 5;
+    if (depth > 64) return bfail(b, EL_ERR_RANGE, "max_depth above 64");
+    if (p->bvh_width != 0 && p->bvh_width != 4 && p->bvh_width != 8) return bfail(b, EL_ERR_ARG, "bvh_width must be 4 or 8");
+    if (p->block != 0 && (p->block % 64 != 0 || p->block > (uint32_t)kMaxBlock))
+        return bfail(b, EL_ERR_ARG, "block must be a multiple of 64, at most 256");
+    if (p->refill > 64) return bfail(b, EL_ERR_ARG, "refill must be 0..64");
+    EL_BHIP(b, hipSetDevice(b->device));
+    const int width = p->bvh_width ? (int)p->bvh_width : b->width;
+    if (width != b->width) {
+        for (auto* c : b->scenes) {
+            int rc = uploadBvh(c, width);
+            if (rc) return bfail(b, rc, el_last_error(c));
+        }
+        b->width = width;
+    }
+    // scene records: shared spill capacity, per-scene key and accumulation
+    int gcap = 1;
+    for (auto* c : b->scenes) gcap = std::max(gcap, c->gcap);
+    std::vector<DevScene> recs(S);
+    std::vector<uint64_t> work(S), npadv(S);
+    uint64_t total = 0;
+    for (size_t i = 0; i < S; i++) {
+        el_ctx* c = b->scenes[i];
+        recs[i] = c->ds;
+        recs[i].gcap = gcap;
+        recs[i].seedLo = (uint32_t)b->seed[i];
+        recs[i].seedHi = (uint32_t)(b->seed[i] >> 32);
+        recs[i].accum = c->dAccum.as<float4>();
+        recs[i].counts = c->dCounts.as<uint32_t>();
+        const uint64_t tx = (c->cam.x_res + 7) / 8, ty = (c->cam.y_res + 7) / 8;
+        npadv[i] = tx * ty * 64;
+        const uint32_t spp = sppCap ? std::min(sppCap, b->spp[i]) : b->spp[i];
+        work[i] = (uint64_t)spp * npadv[i];
+        total += work[i];
+    }
+    EL_BHIP(b, upload(b->dScenes, recs.data(), S * sizeof(DevScene), b->stream));
+    EL_BHIP(b, b->dGstack.ensure((size_t)gcap * b->maxThreads * sizeof(int)));
+    // generation plan
+    uint64_t Q = p->queue_cap;
+    if (!Q) {
+        size_t freeB = 0, totalB = 0;
+        EL_BHIP(b, hipMemGetInfo(&freeB, &totalB));
+        Q = std::max<uint64_t>(1u << 20, freeB / 4 / (6 * 32 + 40));
+    }
+    Q = std::max<uint64_t>(64, (Q / 64) * 64);
+    if (Q > total) Q = total;
+    if (Q > 0x7fffffffull) return bfail(b, EL_ERR_RANGE, "queue capacity above 2^31");
+    std::vector<BatchSeg> segs;
+    std::vector<BatchGen> gens;
+    {
+        size_t sc = 0;
+        uint64_t sceneBase = 0;
+        for (uint64_t gs = 0; gs < total; gs += Q) {
+            const uint64_t ge = std::min(total, gs + Q);
+            BatchGen g{segs.size(), 0, (uint32_t)(ge - gs), 0};
+            uint64_t w = gs;
+            while (w < ge) {
+                while (w >= sceneBase + work[sc]) {
+                    sceneBase += work[sc];
+                    sc++;
+                }
+                const uint64_t segEnd = std::min(ge, sceneBase + work[sc]);
+                BatchSeg s{};
+                s.slotStart = (uint32_t)(w - gs);
+                s.count = (uint32_t)(segEnd - w);
+                s.scene = (uint32_t)sc;
+                s.npad = (uint32_t)npadv[sc];
+                s.wStart = w - sceneBase;
+                s.finStart = g.finTotal;
+                g.finTotal += std::min<uint32_t>(s.count, s.npad);
+                segs.push_back(s);
+                g.nseg++;
+                w = segEnd;
+            }
+            gens.push_back(g);
+        }
+    }
+    EL_BHIP(b, upload(b->dSegs, segs.data(), segs.size() * sizeof(BatchSeg), b->stream));
+    LaunchCfg lc;
+    lc.stream = b->stream;
+    lc.block = p->block ? p->block : kDefaultBlock;
+    lc.shadeBlock = 256;
+    lc.maxThreads = b->maxThreads;
+    lc.refill = p->refill ? p->refill : kDefaultRefill;
+    lc.gstack = b->dGstack.as<int>();
+    const uint64_t segSize = (Q + kXcds - 1) / kXcds + lc.maxThreads / kXcds + kMaxBlock;
+    const uint64_t qcap = std::max<uint64_t>(Q, kXcds * segSize);
+    int rc = batchEnsureWf(b, Q, qcap);
+    if (rc) return rc;
+    WfState q{};
+    q.ray[0] = b->wRay0.as<float4>();
+    q.ray[1] = b->wRay1.as<float4>();
+    q.hits = b->wHits.as<float4>();
+    q.shadowH = b->wShH.as<float4>();
+    q.shadowP = b->wShP.as<float4>();
+    q.pathT = b->wT.as<float4>();
+    q.pathL = b->wL.as<float4>();
+    q.pathPix = b->wPix.as<uint32_t>();
+    q.pathSample = b->wSample.as<uint32_t>();
+    q.pathAov = nullptr;
+    q.counters = b->wCounters.as<uint32_t>();
+    q.segSize = (uint32_t)segSize;
+    q.gstack = lc.gstack;
+    q.gthreads = lc.maxThreads;
+    // clear the accumulations and tallies (outside any captured graph)
+    for (size_t i = 0; i < S; i++) {
+        el_ctx* c = b->scenes[i];
+        const size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+        EL_BHIP(b, hipMemsetAsync(c->dAccum.p, 0, npx * sizeof(float4), b->stream));
+        EL_BHIP(b, hipMemsetAsync(c->dCounts.p, 0, npx * sizeof(uint32_t), b->stream));
+    }
+    unsigned long long* ctr = b->dCounters.as<unsigned long long>();
+    EL_BHIP(b, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, b->stream));
+    const DevScene* dsc = b->dScenes.as<DevScene>();
+    auto launchAll = [&]() -> hipError_t {
+        hipError_t e;
+        for (const BatchGen& g : gens) {
+            const BatchSeg* gseg = b->dSegs.as<BatchSeg>() + g.segOffset;
+            if ((e = launchBtCamera(lc, dsc, gseg, g.nseg, g.count, q)) != hipSuccess) return e;
+            for (uint32_t bo = 0; bo < depth; bo++) {
+                RenderArgs a{};
+                a.maxDepth = depth;
+                a.bounce = bo;
+                const int cur = bo & 1, nxt = cur ^ 1;
+                uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
+                uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
+                if ((e = launchBtExtend(lc, dsc, width, q, q.ray[cur], cntCur, cntNxt, g.count, ctr)) != hipSuccess)
+                    return e;
+                if ((e = launchBtShade(lc, dsc, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, g.count, ctr)) !=
+                    hipSuccess)
+                    return e;
+                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowH, q.counters + kQShadowH, g.count, ctr)) !=
+                    hipSuccess)
+                    return e;
+                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowP, q.counters + kQShadowP, g.count, ctr)) !=
+                    hipSuccess)
+                    return e;
+            }
+            if ((e = launchBtFinalize(lc, dsc, gseg, g.nseg, g.finTotal, q, ctr)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    };
+    const bool useGraph = (p->flags & EL_BATCH_GRAPH) != 0;
+    if (useGraph) {
+        // the captured sequence depends on the plan and the launch shape
+        std::string key = std::to_string(S) + "/" + std::to_string(depth) + "/" + std::to_string(width) + "/" +
+                          std::to_string(lc.block) + "/" + std::to_string(lc.refill) + "/" + std::to_string(Q) +
+                          "/" + std::to_string(sppCap) + "/" + std::to_string(total) + "/" +
+                          std::to_string((uintptr_t)b->dSegs.p) + "/" + std::to_string((uintptr_t)b->wRay0.p) +
+                          "/" + std::to_string((uintptr_t)b->dScenes.p) + "/" + std::to_string((uintptr_t)b->dGstack.p);
+        if (!b->graph || key != b->graphKey) {
+            if (b->graph) {
+                EL_BHIP(b, hipGraphExecDestroy(b->graph));
+                b->graph = nullptr;
+            }
+            EL_BHIP(b, hipStreamSynchronize(b->stream));
+            hipGraph_t gph = nullptr;
+            EL_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launchAll();
+            hipError_t ce = hipStreamEndCapture(b->stream, &gph);
+            if (le != hipSuccess) return bfail(b, EL_ERR_HIP, std::string("capture: ") + hipGetErrorString(le));
+            if (ce != hipSuccess) return bfail(b, EL_ERR_HIP, std::string("end capture: ") + hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&b->graph, gph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gph);
+            if (ie != hipSuccess) {
+                b->graph = nullptr;
+                return bfail(b, EL_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            }
+            b->graphKey = key;
+        }
+        EL_BHIP(b, hipEventRecord(b->e0, b->stream));
+        EL_BHIP(b, hipGraphLaunch(b->graph, b->stream));
+        EL_BHIP(b, hipEventRecord(b->e1, b->stream));
+    } else {
+        EL_BHIP(b, hipEventRecord(b->e0, b->stream));
+        EL_BHIP(b, launchAll());
+        EL_BHIP(b, hipEventRecord(b->e1, b->stream));
+    }
+    unsigned long long h[kCounterWords];
+    EL_BHIP(b, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, b->stream));
+    EL_BHIP(b, hipStreamSynchronize(b->stream));
+    float t = 0;
+    EL_BHIP(b, hipEventElapsedTime(&t, b->e0, b->e1));
+    *ms = t;
+    for (size_t i = 0; i < S; i++) {
+        el_ctx* c = b->scenes[i];
+        c->sampleBase = sppCap ? std::min(sppCap, b->spp[i]) : b->spp[i];
+        c->accSeed = b->seed[i];
+        c->accDepth = depth;
+        c->accMixed = true;   // batch RNG streams carry the scene index: no el_adjoint on them
+    }
+    b->stats = el_stats{};
+    b->stats.rays = tally(h, kStatRays);
+    b->stats.vertices = tally(h, kStatVerts);
+    b->stats.paths = tally(h, kStatPaths);
+    b->stats.render_ms = t;
+    b->stats.bvh_width = (uint32_t)width;
+    b->stats.extend_launches = gens.size() * depth;
+    if (tally(h, kStatOverflow)) return bfail(b, EL_ERR_RANGE, "traversal stack or queue overflow (results incomplete)");
+    return EL_OK;
+}
+
+extern "C" int el_batch_render(el_batch* b, const el_batch_params* p) {
+    if (!b || !p) return bfail(b, EL_ERR_ARG, "null argument");
+    double ms = 0;
+    return batchRender(b, p, 0, &ms);
+}
+
+// Coordinate descent over BVH width, queue capacity, traversal block size
+// and refill threshold, each trial rendering every scene at min(spp,
+// trial_spp) samples; keeps the fastest setting of each knob in turn.
+extern "C" int el_batch_tune(el_batch* b, el_batch_params* io, uint32_t trial_spp) {
+    if (!b || !io) return bfail(b, EL_ERR_ARG, "null argument");
+    if (trial_spp == 0) trial_spp = 2;
+    el_batch_params best = *io;
+    best.flags &= ~EL_BATCH_GRAPH;
+    // explicit defaults, so a sweep value equal to the default is not retried
+    if (!best.bvh_width) best.bvh_width = (uint32_t)b->width;
+    if (!best.block) best.block = kDefaultBlock;
+    if (!best.refill) best.refill = kDefaultRefill;
+    std::ostringstream log;
+    auto trial = [&](const el_batch_params& p, double* ms) -> int {
+        double t = 0;
+        int rc = batchRender(b, &p, trial_spp, &t);   // warm (BVH repack, buffers)
+        if (rc) return rc;
+        rc = batchRender(b, &p, trial_spp, &t);
+        *ms = t;
+        log << "width " << (p.bvh_width ? p.bvh_width : 4) << " queue " << p.queue_cap << " block "
+            << (p.block ? p.block : kDefaultBlock) << " refill " << (p.refill ? p.refill : kDefaultRefill) << ": "
+            << t << " ms\n";
+        return EL_OK;
+    };
+    double bestMs = 0;
+    int rc = trial(best, &bestMs);
+    if (rc) return rc;
+    auto sweep = [&](uint32_t el_batch_params::*field, std::vector<uint32_t> values) -> int {
+        for (uint32_t v : values) {
+            el_batch_params p = best;
+            p.*field = v;
+            if (p.*field == best.*field) continue;
+            double ms = 0;
+            int r = trial(p, &ms);
+            if (r) return r;
+            if (ms < bestMs) {
+                bestMs = ms;
+                best = p;
+            }
+        }
+        return EL_OK;
+    };
+    if ((rc = sweep(&el_batch_params::bvh_width, {4, 8}))) return rc;
+    if ((rc = sweep(&el_batch_params::queue_cap, {1u << 22, 1u << 24, 1u << 26, 0}))) return rc;
+    if ((rc = sweep(&el_batch_params::block, {64, 128, 256}))) return rc;
+    if ((rc = sweep(&el_batch_params::refill, {8, 16, 32, 64}))) return rc;
+    log << "chosen: width " << (best.bvh_width ? best.bvh_width : 4) << " queue " << best.queue_cap << " block "
+        << (best.block ? best.block : kDefaultBlock) << " refill " << (best.refill ? best.refill : kDefaultRefill)
+        << " (" << bestMs << " ms per trial)\n";
+    b->log = log.str();
+    best.flags = io->flags;
+    *io = best;
+    return EL_OK;
+}

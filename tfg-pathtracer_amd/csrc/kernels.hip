@@ -170,7 +170,8 @@ EL_KERNEL(EL_WPE_TRAV) void k_trace(DevScene s, const float* __restrict__ rays, 
     TraceCounters cnt{0, 0};
     uint32_t begin, end, traced = 0;
     waveRange(n, begin, end);
-    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm) {
+    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
+                    BvhPtr&) {
         const float* r = rays + (uint64_t)i * 6;
         ox = r[0];
         oy = r[1];
@@ -582,18 +583,37 @@ EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
         }
         float4* Q = q.ray[0] + 2 * (uint64_t)slot;
         Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
-        Q[1] = make_float4(d.x, d.y, d.z, 0.f);
+        Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(a.scene));
     }
     if (blockIdx.x == 0 && threadIdx.x < kXcds) q.counters[kQRay0 + threadIdx.x] = threadIdx.x == 0 ? a.genCount : 0u;
 }
 
+// Scene addressing of the wavefront kernels.  OneScene: the kernel argument
+// is the scene.  ManyScenes: a device array of scenes (a batch), indexed by
+// the scene id every queue entry carries (ray queue: bits of d.w; shadow
+// queue: bits of the contribution's w).  All scenes of a batch share the BVH
+// width and the traversal stack's spill capacity.
+struct OneScene {
+    DevScene s;
+    static constexpr bool kMany = false;
+    __device__ __forceinline__ const DevScene& get(uint32_t) const { return s; }
+    __device__ __forceinline__ const DevScene& any() const { return s; }
+};
+struct ManyScenes {
+    const DevScene* p;
+    static constexpr bool kMany = true;
+    __device__ __forceinline__ const DevScene& get(uint32_t i) const { return p[i]; }
+    __device__ __forceinline__ const DevScene& any() const { return p[0]; }
+};
+
 // Also clears the counters the following shade kernel appends to: the next
 // ray queue and both shadow queues.
-template <int W, bool COUNT>
-EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(DevScene s, WfState q, const float4* __restrict__ rayIn,
+template <int W, bool COUNT, class P>
+EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restrict__ rayIn,
                                         const uint32_t* countIn, uint32_t* resetNext, uint32_t refill,
                                         unsigned long long* counters) {
-    TravStack st = makeStack(s, q.gstack, q.gthreads);
+    const DevScene& s0 = sc.any();
+    TravStack st = makeStack(s0, q.gstack, q.gthreads);
     TraceCounters cnt{0, 0};
     QueueMap qm;
     qm.load(countIn);
@@ -606,7 +626,8 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(DevScene s, WfState q, const float4* __r
     uint32_t begin, end, rays = 0;
     waveRange(n, begin, end);
     const uint32_t segSize = q.segSize;
-    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm) {
+    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
+                    BvhPtr& bvh) {
         const uint64_t e = qm.entry(i, segSize);
         const float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
         ox = A.x;
@@ -616,18 +637,23 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(DevScene s, WfState q, const float4* __r
         dy = B.y;
         dz = B.z;
         tm = __int_as_float(0x7f800000);
+        if (P::kMany) {
+            const DevScene& sx = sc.get(__float_as_uint(B.w));
+            bvh = BvhPtr{(const uint4*)sx.nodes, sx.tris};
+        }
         return !(B.x == 0.f && B.y == 0.f && B.z == 0.f);   // zero direction: camera-queue padding
     };
     auto store = [&](uint32_t i, const TraceResult& h, bool) {
         q.hits[qm.entry(i, segSize)] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
     };
-    traceWaveRange<W, false, COUNT>(s, st, begin, end, refill, load, store, cnt, rays);
+    traceWaveRange<W, false, COUNT, P::kMany>(s0, st, begin, end, refill, load, store, cnt, rays);
     const int slots[4] = {kStatRays, kStatOverflow, kStatNodes, kStatTris};
     const uint32_t vals[4] = {rays, (uint32_t)st.overflow, cnt.nodes, cnt.tris};
     blockFlush<4>(counters, slots, vals);
 }
 
-EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(DevScene s, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
+template <class P>
+EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
                            const uint32_t* countIn, float4* __restrict__ rayOut, uint32_t* countOut,
                            unsigned long long* counters) {
     QueueMap qm;
@@ -642,12 +668,14 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(DevScene s, RenderArgs a, WfState q, con
         uint32_t i = b + threadIdx.x;
         bool cont = false, wantH = false, wantP = false;
         VertexOut vo;
-        uint32_t slot = 0;
+        uint32_t slot = 0, sid = 0;
         if (i < n) {
             uint64_t e = qm.entry(i, q.segSize);
             float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
             float4 hr = q.hits[e];
             slot = __float_as_uint(A.w);
+            sid = __float_as_uint(B.w);
+            const DevScene& s = sc.get(sid);
             int tri = __float_as_int(hr.x);
             if (tri == -1) {
                 v3 d = mk(B.x, B.y, B.z);
@@ -660,7 +688,12 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(DevScene s, RenderArgs a, WfState q, con
                 float4 Tf = q.pathT[slot];
                 SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
                 HitData hd = hitData(s, sh);
-                PathRng rng{a.k0, a.k1, q.pathPix[slot], q.pathSample[slot], a.scene};
+                uint32_t k0 = a.k0, k1 = a.k1;
+                if (P::kMany) {
+                    k0 = s.seedLo;
+                    k1 = s.seedHi;
+                }
+                PathRng rng{k0, k1, q.pathPix[slot], q.pathSample[slot], sid};
                 uint4 rb = rng.block(2 + bounce);
                 uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
                 vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
@@ -687,19 +720,19 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(DevScene s, RenderArgs a, WfState q, con
         if (at[0] != ~0ull) {
             float4* Q = rayOut + 2 * at[0];
             Q[0] = make_float4(vo.nextO.x, vo.nextO.y, vo.nextO.z, __uint_as_float(slot));
-            Q[1] = make_float4(vo.nextD.x, vo.nextD.y, vo.nextD.z, 0.f);
+            Q[1] = make_float4(vo.nextD.x, vo.nextD.y, vo.nextD.z, __uint_as_float(sid));
         }
         if (at[1] != ~0ull) {
             float4* S = q.shadowH + 3 * at[1];
             S[0] = make_float4(vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, __uint_as_float(slot));
             S[1] = make_float4(vo.hDir.x, vo.hDir.y, vo.hDir.z, __int_as_float(0x7f800000));
-            S[2] = make_float4(vo.contribH.x, vo.contribH.y, vo.contribH.z, 0);
+            S[2] = make_float4(vo.contribH.x, vo.contribH.y, vo.contribH.z, __uint_as_float(sid));
         }
         if (at[2] != ~0ull) {
             float4* S = q.shadowP + 3 * at[2];
             S[0] = make_float4(vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, __uint_as_float(slot));
             S[1] = make_float4(vo.pDir.x, vo.pDir.y, vo.pDir.z, vo.pTmax);
-            S[2] = make_float4(vo.contribP.x, vo.contribP.y, vo.contribP.z, 0);
+            S[2] = make_float4(vo.contribP.x, vo.contribP.y, vo.contribP.z, __uint_as_float(sid));
         }
     }
     const int slots[3] = {kStatRays, kStatVerts, kStatOverflow};
@@ -710,11 +743,12 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(DevScene s, RenderArgs a, WfState q, con
 // Any-hit test of a shadow queue; an unoccluded entry adds its contribution
 // to its path's radiance (each path has at most one entry per queue, and the
 // two shadow queues are separate launches, so the update needs no atomics).
-template <int W>
-EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(DevScene s, WfState q, const float4* __restrict__ sh,
+template <int W, class P>
+EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restrict__ sh,
                                         const uint32_t* countIn, uint32_t refill,
                                         unsigned long long* counters) {
-    TravStack st = makeStack(s, q.gstack, q.gthreads);
+    const DevScene& s0 = sc.any();
+    TravStack st = makeStack(s0, q.gstack, q.gthreads);
     TraceCounters cnt{0, 0};
     QueueMap qm;
     qm.load(countIn);
@@ -722,7 +756,8 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(DevScene s, WfState q, const float4* __r
     uint32_t begin, end, traced = 0;
     waveRange(n, begin, end);
     const uint32_t segSize = q.segSize;
-    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm) {
+    auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
+                    BvhPtr& bvh) {
         const uint64_t e = qm.entry(i, segSize);
         const float4 A = sh[3 * e], B = sh[3 * e + 1];
         ox = A.x;
@@ -732,6 +767,10 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(DevScene s, WfState q, const float4* __r
         dy = B.y;
         dz = B.z;
         tm = B.w;
+        if (P::kMany) {
+            const DevScene& sx = sc.get(__float_as_uint(sh[3 * e + 2].w));
+            bvh = BvhPtr{(const uint4*)sx.nodes, sx.tris};
+        }
         return true;
     };
     auto store = [&](uint32_t i, const TraceResult& h, bool) {
@@ -742,7 +781,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(DevScene s, WfState q, const float4* __r
         const float4 L = q.pathL[slot];
         q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
     };
-    traceWaveRange<W, true, false>(s, st, begin, end, refill, load, store, cnt, traced);
+    traceWaveRange<W, true, false, P::kMany>(s0, st, begin, end, refill, load, store, cnt, traced);
     const int slots[1] = {kStatOverflow};
     const uint32_t vals[1] = {(uint32_t)st.overflow};
     blockFlush<1>(counters, slots, vals);
@@ -772,6 +811,81 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
                 const float4* P = q.pathAov + 3 * (uint64_t)slot;
                 for (int k = 0; k < 3; k++) A[k] = make_float4(A[k].x + P[k].x, A[k].y + P[k].y, A[k].z + P[k].z, 0);
             }
+        }
+    }
+    const int slots[1] = {kStatPaths};
+    const uint32_t vals[1] = {paths};
+    blockFlush<1>(counters, slots, vals);
+}
+
+// ------------------------------------------------------------------ batch --
+// Segment of a generation slot (or finalize thread): the last segment whose
+// start is <= t.  nseg is small (at most the number of scenes + 1).
+__device__ __forceinline__ uint32_t findSeg(const BatchSeg* segs, uint32_t nseg, uint32_t t, bool fin) {
+    uint32_t lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if ((fin ? segs[mid].finStart : segs[mid].slotStart) <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Camera rays of a batch generation: slot -> (scene, sample, pixel) through
+// the segment table; the scene id rides in the ray's d.w.
+EL_KERNEL(1) void k_bt_camera(ManyScenes sc, const BatchSeg* __restrict__ segs, uint32_t nseg, uint32_t genCount,
+                              WfState q) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < genCount; slot += stride) {
+        const BatchSeg g = segs[findSeg(segs, nseg, slot, false)];
+        const DevScene& s = sc.get(g.scene);
+        const uint64_t w = g.wStart + (slot - g.slotStart);
+        const uint32_t p = (uint32_t)(w % g.npad);
+        const uint32_t sample = (uint32_t)(w / g.npad);
+        uint32_t x, y;
+        const bool valid = workPixel(s, p, x, y);
+        const uint32_t idx = valid ? s.cam.xRes * (s.cam.yRes - y - 1) + x : 0xFFFFFFFFu;
+        q.pathT[slot] = make_float4(1, 1, 1, 0);
+        q.pathL[slot] = make_float4(0, 0, 0, valid ? 1.f : 0.f);
+        q.pathPix[slot] = idx;
+        q.pathSample[slot] = sample;
+        v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+        if (valid) {
+            PathRng rng{s.seedLo, s.seedHi, idx, sample, g.scene};
+            uint4 c = rng.block(0);
+            uint32_t c5 = rng.block(1).x;
+            cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
+        }
+        float4* Q = q.ray[0] + 2 * (uint64_t)slot;
+        Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
+        Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(g.scene));
+    }
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) q.counters[kQRay0 + threadIdx.x] = threadIdx.x == 0 ? genCount : 0u;
+}
+
+// Accumulation of a batch generation: one thread per distinct (scene, pixel)
+// of the generation (slots j, j + npad, ... of a segment share the pixel),
+// so no two threads touch one pixel and no atomics are needed.
+EL_KERNEL(1) void k_bt_finalize(ManyScenes sc, const BatchSeg* __restrict__ segs, uint32_t nseg, uint32_t finTotal,
+                                WfState q, unsigned long long* counters) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    uint32_t paths = 0;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < finTotal; t += stride) {
+        const BatchSeg g = segs[findSeg(segs, nseg, t, true)];
+        const DevScene& s = sc.get(g.scene);
+        const uint32_t end = g.slotStart + g.count;
+        for (uint32_t slot = g.slotStart + (t - g.finStart); slot < end; slot += g.npad) {
+            const float4 Lf = q.pathL[slot];
+            if (Lf.w == 0.f) continue;   // padding pixel
+            paths++;
+            const uint32_t idx = q.pathPix[slot];
+            const v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
+            if (anyNan(L)) continue;
+            const float4 acc = s.accum[idx];
+            s.accum[idx] = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
+            s.counts[idx] += 1;
         }
     }
     const int slots[1] = {kStatPaths};
@@ -858,8 +972,8 @@ hipError_t launchWfExtend(const LaunchCfg& c, const DevScene& s, const WfState& 
     uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
 #define EL_EXT(WW, C)                                                                                       \
-    hipLaunchKernelGGL((k_wf_extend<WW, C>), dim3(grid), dim3(c.block), lds, c.stream, s, q, rayIn, countIn, \
-                       resetNext, c.refill, counters)
+    hipLaunchKernelGGL((k_wf_extend<WW, C, OneScene>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s}, q, \
+                       rayIn, countIn, resetNext, c.refill, counters)
     if (s.width == 4) {
         if (countTraversal)
             EL_EXT(4, true);
@@ -879,8 +993,8 @@ hipError_t launchWfShade(const LaunchCfg& c, const DevScene& s, const RenderArgs
                          const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
                          uint32_t maxCount, unsigned long long* counters) {
     uint32_t grid = gridFor(maxCount, c.shadeBlock, c.maxThreads / c.shadeBlock);
-    hipLaunchKernelGGL(k_wf_shade, dim3(grid), dim3(c.shadeBlock), 0, c.stream, s, a, q, rayIn, countIn, rayOut,
-                       countOut, counters);
+    hipLaunchKernelGGL(k_wf_shade<OneScene>, dim3(grid), dim3(c.shadeBlock), 0, c.stream, OneScene{s}, a, q, rayIn,
+                       countIn, rayOut, countOut, counters);
     return hipGetLastError();
 }
 
@@ -889,11 +1003,11 @@ hipError_t launchWfShadow(const LaunchCfg& c, const DevScene& s, const WfState& 
     uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
     if (s.width == 4)
-        hipLaunchKernelGGL(k_wf_shadow<4>, dim3(grid), dim3(c.block), lds, c.stream, s, q, shadow, countIn,
-                           c.refill, counters);
+        hipLaunchKernelGGL((k_wf_shadow<4, OneScene>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s}, q,
+                           shadow, countIn, c.refill, counters);
     else
-        hipLaunchKernelGGL(k_wf_shadow<8>, dim3(grid), dim3(c.block), lds, c.stream, s, q, shadow, countIn,
-                           c.refill, counters);
+        hipLaunchKernelGGL((k_wf_shadow<8, OneScene>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s}, q,
+                           shadow, countIn, c.refill, counters);
     return hipGetLastError();
 }
 
@@ -903,6 +1017,60 @@ hipError_t launchWfFinalize(const LaunchCfg& c, const DevScene& s, const RenderA
     uint32_t grid = gridFor(lim, c.shadeBlock, c.maxThreads / c.shadeBlock);
     hipLaunchKernelGGL(k_wf_finalize, dim3(grid), dim3(c.shadeBlock), 0, c.stream, s, a, q, accum, counts, aov,
                        counters);
+    return hipGetLastError();
+}
+
+// ---- batch launchers
+hipError_t launchBtCamera(const LaunchCfg& c, const DevScene* scenes, const BatchSeg* segs, uint32_t nseg,
+                          uint32_t genCount, const WfState& q) {
+    uint32_t grid = gridFor(genCount, c.shadeBlock, c.maxThreads / c.shadeBlock);
+    hipLaunchKernelGGL(k_bt_camera, dim3(grid), dim3(c.shadeBlock), 0, c.stream, ManyScenes{scenes}, segs, nseg,
+                       genCount, q);
+    return hipGetLastError();
+}
+
+hipError_t launchBtExtend(const LaunchCfg& c, const DevScene* scenes, int width, const WfState& q,
+                          const float4* rayIn, const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount,
+                          unsigned long long* counters) {
+    uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+    if (width == 4)
+        hipLaunchKernelGGL((k_wf_extend<4, false, ManyScenes>), dim3(grid), dim3(c.block), lds, c.stream,
+                           ManyScenes{scenes}, q, rayIn, countIn, resetNext, c.refill, counters);
+    else
+        hipLaunchKernelGGL((k_wf_extend<8, false, ManyScenes>), dim3(grid), dim3(c.block), lds, c.stream,
+                           ManyScenes{scenes}, q, rayIn, countIn, resetNext, c.refill, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchBtShade(const LaunchCfg& c, const DevScene* scenes, const RenderArgs& a, const WfState& q,
+                         const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
+                         uint32_t maxCount, unsigned long long* counters) {
+    uint32_t grid = gridFor(maxCount, c.shadeBlock, c.maxThreads / c.shadeBlock);
+    hipLaunchKernelGGL(k_wf_shade<ManyScenes>, dim3(grid), dim3(c.shadeBlock), 0, c.stream, ManyScenes{scenes}, a, q,
+                       rayIn, countIn, rayOut, countOut, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchBtShadow(const LaunchCfg& c, const DevScene* scenes, int width, const WfState& q,
+                          const float4* shadow, const uint32_t* countIn, uint32_t maxCount,
+                          unsigned long long* counters) {
+    uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
+    size_t lds = sizeof(int) * kShortStack * c.block;
+    if (width == 4)
+        hipLaunchKernelGGL((k_wf_shadow<4, ManyScenes>), dim3(grid), dim3(c.block), lds, c.stream,
+                           ManyScenes{scenes}, q, shadow, countIn, c.refill, counters);
+    else
+        hipLaunchKernelGGL((k_wf_shadow<8, ManyScenes>), dim3(grid), dim3(c.block), lds, c.stream,
+                           ManyScenes{scenes}, q, shadow, countIn, c.refill, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchBtFinalize(const LaunchCfg& c, const DevScene* scenes, const BatchSeg* segs, uint32_t nseg,
+                            uint32_t finTotal, const WfState& q, unsigned long long* counters) {
+    uint32_t grid = gridFor(finTotal, c.shadeBlock, c.maxThreads / c.shadeBlock);
+    hipLaunchKernelGGL(k_bt_finalize, dim3(grid), dim3(c.shadeBlock), 0, c.stream, ManyScenes{scenes}, segs, nseg,
+                       finTotal, q, counters);
     return hipGetLastError();
 }
 

@@ -59,6 +59,16 @@ struct WfState {
     uint32_t gthreads;
 };
 
+// One scene's share of a batch generation: path slots [slotStart,
+// slotStart + count) hold the scene's work items wStart .. (sample-major:
+// w = sample * npad + padded pixel).  finStart: prefix over the generation's
+// segments of min(count, npad), the finalize threads of the segment.
+struct BatchSeg {
+    uint32_t slotStart, count, scene, npad;
+    uint64_t wStart;
+    uint32_t finStart, pad;
+};
+
 struct LaunchCfg {
     hipStream_t stream;
     uint32_t block;         // traversal kernels (trace, extend, shadow) and the plain megakernel
@@ -76,6 +86,21 @@ hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderA
 hipError_t launchAdjoint(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, uint32_t spp,
                          const float* dimage, const uint32_t* counts, int gradTex, double* dmat, float* dtex,
                          unsigned long long* counters);
+// Batch (many scenes, scene id per queue entry): scenes = device array of
+// DevScene; segs = the generation's segment table (device).
+hipError_t launchBtCamera(const LaunchCfg& c, const dev::DevScene* scenes, const BatchSeg* segs, uint32_t nseg,
+                          uint32_t genCount, const WfState& q);
+hipError_t launchBtExtend(const LaunchCfg& c, const dev::DevScene* scenes, int width, const WfState& q,
+                          const float4* rayIn, const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount,
+                          unsigned long long* counters);
+hipError_t launchBtShade(const LaunchCfg& c, const dev::DevScene* scenes, const RenderArgs& a, const WfState& q,
+                         const float4* rayIn, const uint32_t* countIn, float4* rayOut, uint32_t* countOut,
+                         uint32_t maxCount, unsigned long long* counters);
+hipError_t launchBtShadow(const LaunchCfg& c, const dev::DevScene* scenes, int width, const WfState& q,
+                          const float4* shadow, const uint32_t* countIn, uint32_t maxCount,
+                          unsigned long long* counters);
+hipError_t launchBtFinalize(const LaunchCfg& c, const dev::DevScene* scenes, const BatchSeg* segs, uint32_t nseg,
+                            uint32_t finTotal, const WfState& q, unsigned long long* counters);
 hipError_t launchWfCamera(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, const WfState& q);
 hipError_t launchWfExtend(const LaunchCfg& c, const dev::DevScene& s, const WfState& q, const float4* rayIn,
                           const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount, bool countTraversal,

@@ -139,6 +139,10 @@ struct DevScene {
     int32_t width;                 // BVH width (4 or 8)
     int32_t gcap;                  // global stack entries per thread
     DevCamera cam;
+    // batches only: the scene's RNG key and accumulation buffers
+    uint32_t seedLo, seedHi;
+    float4* accum;
+    uint32_t* counts;
 };
 
 // --------------------------------------------------------------- textures --
@@ -811,15 +815,22 @@ __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, flo
 // hit is order-independent (t, then lower index), so results are identical
 // to traceRay.
 //
-//   load(i, ox, oy, oz, dx, dy, dz, tmax) -> false if item i has no ray
-//   store(i, result, traced)              once per item
+//   load(i, ox, oy, oz, dx, dy, dz, tmax, bvh) -> false if item i has no ray;
+//                                   with PER_RAY_BVH it also sets the BVH
+//                                   (nodes, triangles) the ray traverses
+//   store(i, result, traced)        once per item
 //
 // Every lane of the wave must call with the same begin/end.
-template <int W, bool ANY, bool COUNT, class Load, class Store>
+struct BvhPtr {
+    const uint4* nodes;
+    const float4* tris;
+};
+
+template <int W, bool ANY, bool COUNT, bool PER_RAY_BVH = false, class Load, class Store>
 __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st, uint32_t begin, uint32_t end,
                                                uint32_t refill, Load&& load, Store&& store, TraceCounters& cnt,
                                                uint32_t& traced) {
-    const uint4* __restrict__ nodes = (const uint4*)s.nodes;
+    BvhPtr bvh{(const uint4*)s.nodes, s.tris};
     const uint32_t lane = __lane_id();
     const unsigned long long ltMask = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint32_t cursor = begin;   // wave-uniform
@@ -837,7 +848,9 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                 uint32_t i = cursor + (uint32_t)__popcll(idle & ltMask);
                 if (i < end) {
                     float ox, oy, oz, dx, dy, dz, tmax;
-                    if (load(i, ox, oy, oz, dx, dy, dz, tmax)) {
+                    BvhPtr rb = bvh;
+                    if (load(i, ox, oy, oz, dx, dy, dz, tmax, rb)) {
+                        if (PER_RAY_BVH) bvh = rb;
                         ray = makeTRay(ox, oy, oz, dx, dy, dz);
                         r.tri = -1;
                         r.t = tmax;
@@ -866,7 +879,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
             if (__all(!act || leaf != -1)) break;   // also when no lane can descend
             if (act) {
                 if (COUNT) cnt.nodes++;
-                cur = visitNode<W>(nodes, cur, ray, cullT<ANY>(r.t), st);
+                cur = visitNode<W>(PER_RAY_BVH ? bvh.nodes : (const uint4*)s.nodes, cur, ray, cullT<ANY>(r.t), st);
                 if (cur <= -2 && leaf == -1) {
                     leaf = cur;
                     cur = st.pop();
@@ -886,7 +899,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                     code = cur;
                     cur = st.pop();
                 }
-                if (testLeaf<ANY, COUNT>(s.tris, code, ray, r, cnt)) {
+                if (testLeaf<ANY, COUNT>(PER_RAY_BVH ? bvh.tris : s.tris, code, ray, r, cnt)) {
                     cur = -1;   // any-hit: done
                     leaf = -1;
                 }

@@ -77,7 +77,12 @@ class el_scene_view(C.Structure):
 class el_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("seed", C.c_uint64), ("mode", C.c_int32),
                 ("block", C.c_uint32), ("queue_cap", C.c_uint32), ("bvh_width", C.c_uint32),
-                ("flags", C.c_uint32), ("refill", C.c_uint32)]
+                ("flags", C.c_uint32), ("refill", C.c_uint32), ("rng_scene", C.c_uint32)]
+
+
+class el_batch_params(C.Structure):
+    _fields_ = [("max_depth", C.c_uint32), ("block", C.c_uint32), ("queue_cap", C.c_uint32),
+                ("bvh_width", C.c_uint32), ("refill", C.c_uint32), ("flags", C.c_uint32)]
 
 
 class el_stats(C.Structure):
@@ -116,6 +121,15 @@ _SIGS = {
     "el_set_materials": (C.c_int, [_P, C.POINTER(el_material), C.c_uint32]),
     "el_set_texture": (C.c_int, [_P, C.c_int32, _P]),
     "el_adjoint": (C.c_int, [_P, C.POINTER(el_adjoint_params), _P, _P, _P, C.c_int]),
+    "el_batch_create": (C.c_int, [C.POINTER(_P), C.c_int]),
+    "el_batch_destroy": (None, [_P]),
+    "el_batch_last_error": (C.c_char_p, [_P]),
+    "el_batch_log": (C.c_char_p, [_P]),
+    "el_batch_add": (C.c_int, [_P, C.POINTER(el_scene_view), C.c_uint32, C.c_uint64, C.POINTER(C.c_int32)]),
+    "el_batch_render": (C.c_int, [_P, C.POINTER(el_batch_params)]),
+    "el_batch_tune": (C.c_int, [_P, C.POINTER(el_batch_params), C.c_uint32]),
+    "el_batch_read": (C.c_int, [_P, C.c_int32, _P, _P]),
+    "el_batch_stats": (C.c_int, [_P, C.POINTER(el_stats)]),
     "el_trace_closest": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, C.c_int]),
     "el_trace_any": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_int]),
     "el_trace_count": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int]),
@@ -338,8 +352,8 @@ class Context:
         self.scene = scene
 
     def render(self, spp, depth=5, seed=0, mode=MODE_WAVEFRONT, block=0, queue_cap=0, bvh_width=0, flags=0,
-               refill=0):
-        p = el_render_params(spp, depth, seed, mode, block, queue_cap, bvh_width, flags, refill)
+               refill=0, rng_scene=0):
+        p = el_render_params(spp, depth, seed, mode, block, queue_cap, bvh_width, flags, refill, rng_scene)
         self._check(lib().el_render(self._h, C.byref(p)))
         return self.stats()
 
@@ -436,3 +450,64 @@ class Context:
         ms = C.c_double()
         self._check(lib().el_trace_bench(self._h, d_rays, n, d_tri, d_t, iters, C.byref(ms)))
         return ms.value
+
+
+BATCH_GRAPH = 1
+
+
+class Batch:
+    """Many scenes through one wavefront loop (el_batch_*)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().el_batch_create(C.byref(h), device)
+        if rc != EL_OK:
+            raise ElevenError(rc, "el_batch_create failed (no HIP device?)")
+        self._h = h
+        self.scenes = []
+
+    def _check(self, rc):
+        if rc != EL_OK:
+            raise ElevenError(rc, lib().el_batch_last_error(self._h).decode())
+
+    def add(self, scene: SceneData, spp: int, seed: int = 0) -> int:
+        idx = C.c_int32()
+        self._check(lib().el_batch_add(self._h, C.byref(scene.view), spp, seed, C.byref(idx)))
+        self.scenes.append(scene)
+        return idx.value
+
+    def render(self, depth=5, block=0, queue_cap=0, bvh_width=0, refill=0, graph=False) -> el_stats:
+        p = el_batch_params(depth, block, queue_cap, bvh_width, refill, BATCH_GRAPH if graph else 0)
+        self._check(lib().el_batch_render(self._h, C.byref(p)))
+        return self.stats()
+
+    def tune(self, depth=5, trial_spp=2, graph=False) -> dict:
+        p = el_batch_params(depth, 0, 0, 0, 0, BATCH_GRAPH if graph else 0)
+        self._check(lib().el_batch_tune(self._h, C.byref(p), trial_spp))
+        return {"block": p.block, "queue_cap": p.queue_cap, "bvh_width": p.bvh_width, "refill": p.refill}
+
+    def log(self) -> str:
+        return lib().el_batch_log(self._h).decode()
+
+    def read(self, i: int):
+        if not 0 <= i < len(self.scenes):
+            raise ElevenError(EL_ERR_ARG, f"scene index {i} out of range (batch of {len(self.scenes)})")
+        cam = self.scenes[i].camera
+        rgba = np.zeros((cam.y_res, cam.x_res, 4), np.float32)
+        samples = np.zeros((cam.y_res, cam.x_res), np.uint32)
+        self._check(lib().el_batch_read(self._h, i, rgba.ctypes.data, samples.ctypes.data))
+        return rgba, samples
+
+    def stats(self) -> el_stats:
+        s = el_stats()
+        self._check(lib().el_batch_stats(self._h, C.byref(s)))
+        return s
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().el_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()

@@ -237,6 +237,72 @@ def sphere_field(out_dir, n_spheres=64, res=(256, 256), seed=0, seg=32, rings=16
     return out
 
 
+def sphere_field_arrays(n_spheres=64, res=(256, 256), seed=0, seg=32, rings=16, hdri=None, env=0.5, lights=2,
+                        light_radiance=50.0, extent=10.0):
+    """The sphere field of sphere_field(), built in memory as a SceneData (no
+    files): world-space triangles, the same material parameters (albedo ~
+    U[0.1,0.9]^3, roughness ~ U[0.05,1], metallic ~ Bernoulli(0.3), specular
+    0.5), lights and camera.  `hdri`: float RGB array or None for a constant
+    environment `env`."""
+    from . import SceneData, make_material
+    rng = np.random.default_rng(seed)
+    g = 20.0
+    verts, nrms, uvs, oid = [], [], [], []
+    gv = np.array([(-g, 0, -g), (g, 0, -g), (g, 0, g), (-g, 0, g)], np.float64)
+    guv = np.array([(0, 0), (1, 0), (1, 1), (0, 1)], np.float64)
+    for f in ((0, 1, 2), (0, 2, 3)):
+        verts.append(gv[list(f)].reshape(9))
+        nrms.append(np.tile([0.0, 1.0, 0.0], 3))
+        uvs.append(guv[list(f)].reshape(6))
+        oid.append(0)
+    mats = [make_material(albedo=(0.5, 0.5, 0.5), roughness=1.0, metallic=0.0, specular=0.5)]
+    centers = rng.uniform(-extent, extent, size=(n_spheres, 2))
+    kd = rng.uniform(0.1, 0.9, size=(n_spheres, 3))
+    pr = rng.uniform(0.05, 1.0, size=n_spheres)
+    pm = (rng.uniform(size=n_spheres) < 0.3).astype(np.float64)
+    V, UV, N, F = uv_sphere((0.0, 0.0, 0.0), 1.0, seg, rings)
+    tv, tn, tu = V[F].reshape(-1, 9), N[F].reshape(-1, 9), UV[F].reshape(-1, 6)
+    for i in range(n_spheres):
+        off = np.tile([centers[i, 0], 1.0, centers[i, 1]], 3)
+        verts.append(tv + off)
+        nrms.append(tn)
+        uvs.append(tu)
+        oid.append(np.full(len(tv), i + 1))
+        mats.append(make_material(albedo=tuple(kd[i]), roughness=float(pr[i]), metallic=float(pm[i]), specular=0.5))
+    v = np.vstack([np.atleast_2d(x) for x in verts]).astype(np.float32)
+    n = np.vstack([np.atleast_2d(x) for x in nrms]).astype(np.float32)
+    uv = np.vstack([np.atleast_2d(x) for x in uvs]).astype(np.float32)
+    ob = np.concatenate([np.atleast_1d(x) for x in oid]).astype(np.int32)
+    lp = [((float(rng.uniform(-extent, extent)), float(rng.uniform(5, 10)), float(rng.uniform(-extent, extent))),
+           (light_radiance,) * 3) for _ in range(lights)]
+    pitch = math.degrees(math.atan2(8.0, 25.0))
+    cam = dict(x_res=res[0], y_res=res[1], position=(0.0, 8.0, -25.0), rotation=(pitch, 0.0, 0.0),
+               focus_distance=26.0)
+    env_img = hdri if hdri is not None else np.full((4, 8, 3), env, np.float32)
+    return SceneData.from_arrays(v, cam, normals=n, uvs=uv, object_id=ob, mesh_material=np.arange(len(mats)),
+                                 materials=mats, hdri=env_img, lights=lp)
+
+
+def batch_workload(n_scenes=32, seed=0, tri_range=(1000, 500000), side_range=(128, 1024), spp_range=(4, 64)):
+    """Seeded batch of sphere fields: triangle counts log-uniform in tri_range
+    (960 per sphere), square resolutions with side uniform in side_range,
+    spp uniform in spp_range, each with its own 512x256 HDR sky (sun
+    direction per scene).  Returns [(SceneData, spp, seed), ...]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_scenes):
+        tris = math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))
+        n_sph = max(1, int(round((tris - 2) / 960)))
+        side = int(rng.integers(side_range[0], side_range[1] + 1))
+        spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+        sun = (float(rng.uniform(-1, 1)), float(rng.uniform(0.3, 1.0)), float(rng.uniform(-1, 1)))
+        sky = sky_hdr(512, 256, sun_dir=sun, peak=200.0, sigma_deg=3.0)
+        ext = 10.0 * max(1.0, math.sqrt(n_sph / 64.0))
+        sc = sphere_field_arrays(n_spheres=n_sph, res=(side, side), seed=seed * 1000 + i, hdri=sky, extent=ext)
+        out.append((sc, spp, seed * 1000 + i))
+    return out
+
+
 def value_noise(x, z, seed, octaves=6):
     """sum_o 2^-o * valuenoise(2^o p) with a seeded lattice and smoothstep."""
     rng = np.random.default_rng(seed)
