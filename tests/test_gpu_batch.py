@@ -117,3 +117,19 @@ def test_batch_errors():
     with pytest.raises(eleven.ElevenError):
         b.read(5)
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_culling_is_exact(batch_scenes):
+    b = _batch(batch_scenes)
+    b.render(depth=6)
+    culled = [b.read(i) for i in range(len(batch_scenes))]
+    r_cull = b.stats().rays
+    b.render(depth=6, cull=False)
+    r_all = b.stats().rays
+    for i in range(len(batch_scenes)):
+        img, cnt = b.read(i)
+        np.testing.assert_array_equal(cnt, culled[i][1])
+        np.testing.assert_array_equal(img, culled[i][0])
+    assert r_cull <= r_all
+    b.close()

@@ -63,7 +63,10 @@ def test_gpu_matches_oracle_common_random_numbers(gpu_ctx, scenes, refbvhs, kind
     print(f"[render] GPU vs oracle, common random numbers: {kind}", flush=True)
     s = scenes[kind]
     spp, seed, depth = 8, 5, 5
-    g, gc, grays, gverts, _ = _render(gpu_ctx, s, spp, seed, eleven.MODE_WAVEFRONT, depth=depth)
+    # faithful mode (every path traced to the end, as the oracle does), so the
+    # ray and vertex tallies are comparable too
+    g, gc, grays, gverts, _ = _render(gpu_ctx, s, spp, seed, eleven.MODE_WAVEFRONT, depth=depth,
+                                      flags=eleven.FLAG_NO_CULL)
     osum, _, ocnt, orays, overts = orc.render(s, refbvhs[kind], spp, depth=depth, seed=seed, precision=0,
                                                 shadow_mode=1)
     om = osum / np.maximum(ocnt, 1)[..., None]
@@ -173,3 +176,21 @@ def test_progressive_accumulation(gpu_ctx, scenes):
     b, cb, rb, _ = gpu_ctx.read()
     assert np.array_equal(ca, cb) and ra == rb
     np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", SCENES)
+def test_zero_throughput_culling_is_exact(gpu_ctx, scenes, kind):
+    """A path whose throughput is exactly zero stops unless a later bounce
+    could make it NaN (which rejects the sample): the image and the
+    accepted-sample counts are bit-identical to tracing every path to the
+    end as the reference does, for both renderers; only the traced-ray
+    count drops."""
+    print(f"[render] culling exactness: {kind}", flush=True)
+    s = scenes[kind]
+    for mode in (eleven.MODE_PLAIN, eleven.MODE_WAVEFRONT):
+        a, ca, ra, _, _ = _render(gpu_ctx, s, 8, 7, mode, depth=8)
+        b, cb, rb, _, _ = _render(gpu_ctx, s, 8, 7, mode, depth=8, flags=eleven.FLAG_NO_CULL)
+        assert np.array_equal(ca, cb)
+        assert np.array_equal(a, b)
+        assert ra <= rb

@@ -88,9 +88,9 @@ struct el_ctx {
     std::vector<dev::DevMaterial> mats;
     std::vector<dev::DevTex> texs;
 
-    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap;
+    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad;
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
-    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wAov, wCounters;
+    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters;
     size_t wfCap = 0, wfQcap = 0;
     bool wfAov = false;
 
@@ -339,6 +339,9 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     EL_HIP(c, upload(c->dEnvCdf, hs.cdf.data(), hs.cdf.size() * sizeof(float), c->stream));
     EL_HIP(c, upload(c->dEnvGuide, hs.guide.data(), hs.guide.size() * sizeof(int32_t), c->stream));
     EL_HIP(c, upload(c->dEnvMap, hs.map2.data(), hs.map2.size() * sizeof(int32_t), c->stream));
+    const std::vector<uint32_t> badBits =
+        hdriBadBuckets(hs, h.data, h.width, h.height, h.x_tile, h.y_tile, h.x_offset, h.y_offset);
+    EL_HIP(c, upload(c->dEnvBad, badBits.data(), badBits.size() * sizeof(uint32_t), c->stream));
 
     // accumulation
     size_t npx = (size_t)v->camera.x_res * v->camera.y_res;
@@ -355,6 +358,7 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     s.envCdf = c->dEnvCdf.as<float>();
     s.envGuide = c->dEnvGuide.as<int32_t>();
     s.envMap = c->dEnvMap.as<int32_t>();
+    s.envBad = c->dEnvBad.as<uint32_t>();
     s.env = env;
     s.envRadianceSum = hs.radianceSum;
     s.envLog2G = hs.log2G;
@@ -420,6 +424,7 @@ static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
     EL_HIP(c, c->wL.ensure(cap * 16));
     EL_HIP(c, c->wPix.ensure(cap * 4));
     EL_HIP(c, c->wSample.ensure(cap * 4));
+    EL_HIP(c, c->wQuiet.ensure(cap * 4));
     if (aov) EL_HIP(c, c->wAov.ensure(cap * 48));
     EL_HIP(c, c->wCounters.ensure(kQCounters * sizeof(uint32_t)));
     c->wfCap = std::max(c->wfCap, cap);
@@ -463,6 +468,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     a.scene = p->rng_scene;
     a.maxDepth = depth;
     a.sample = c->sampleBase;
+    a.cull = (p->flags & EL_FLAG_NO_CULL) ? 0u : 1u;
 
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
     if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
@@ -487,7 +493,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         if (!Q) {
             size_t freeB = 0, totalB = 0;
             EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
-            const uint64_t perPath = 6 * 32 + 40 + (aov ? 48 : 0);
+            const uint64_t perPath = 6 * 32 + 44 + (aov ? 48 : 0);
             Q = std::max<uint64_t>(1u << 20, freeB / 4 / perPath);
         }
         if (Q >= npad)
@@ -513,6 +519,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         q.pathL = c->wL.as<float4>();
         q.pathPix = c->wPix.as<uint32_t>();
         q.pathSample = c->wSample.as<uint32_t>();
+        q.pathQuiet = c->wQuiet.as<uint32_t>();
         q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
         q.counters = c->wCounters.as<uint32_t>();
         q.segSize = (uint32_t)segSize;
@@ -632,6 +639,7 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     a.scene = c->accScene;
     a.maxDepth = c->accDepth;
     a.sample = 0;
+    a.cull = 1;   // exact: a zero-throughput path adds neither radiance nor gradient
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
     if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
     EL_HIP(c, hipEventRecord(e0, c->stream));
@@ -911,7 +919,7 @@ struct el_batch {
     std::vector<uint64_t> seed;
     int width = 4;
     DevBuf dScenes, dSegs, dGstack, dCounters;
-    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wCounters;
+    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wCounters;
     size_t wfCap = 0, wfQcap = 0;
     hipGraphExec_t graph = nullptr;
     std::string graphKey;
@@ -1030,6 +1038,7 @@ static int batchEnsureWf(el_batch* b, size_t cap, size_t qcap) {
     EL_BHIP(b, b->wL.ensure(cap * 16));
     EL_BHIP(b, b->wPix.ensure(cap * 4));
     EL_BHIP(b, b->wSample.ensure(cap * 4));
+    EL_BHIP(b, b->wQuiet.ensure(cap * 4));
     EL_BHIP(b, b->wCounters.ensure(kQCounters * sizeof(uint32_t)));
     b->wfCap = std::max(b->wfCap, cap);
     b->wfQcap = std::max(b->wfQcap, qcap);
@@ -1083,7 +1092,7 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
     if (!Q) {
         size_t freeB = 0, totalB = 0;
         EL_BHIP(b, hipMemGetInfo(&freeB, &totalB));
-        Q = std::max<uint64_t>(1u << 20, freeB / 4 / (6 * 32 + 40));
+        Q = std::max<uint64_t>(1u << 20, freeB / 4 / (6 * 32 + 44));
     }
     Q = std::max<uint64_t>(64, (Q / 64) * 64);
     if (Q > total) Q = total;
@@ -1140,6 +1149,7 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
     q.pathL = b->wL.as<float4>();
     q.pathPix = b->wPix.as<uint32_t>();
     q.pathSample = b->wSample.as<uint32_t>();
+    q.pathQuiet = b->wQuiet.as<uint32_t>();
     q.pathAov = nullptr;
     q.counters = b->wCounters.as<uint32_t>();
     q.segSize = (uint32_t)segSize;
@@ -1159,11 +1169,14 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
         hipError_t e;
         for (const BatchGen& g : gens) {
             const BatchSeg* gseg = b->dSegs.as<BatchSeg>() + g.segOffset;
-            if ((e = launchBtCamera(lc, dsc, gseg, g.nseg, g.count, q)) != hipSuccess) return e;
+            if ((e = launchBtCamera(lc, dsc, gseg, g.nseg, g.count, depth, !(p->flags & EL_BATCH_NO_CULL), q)) !=
+                hipSuccess)
+                return e;
             for (uint32_t bo = 0; bo < depth; bo++) {
                 RenderArgs a{};
                 a.maxDepth = depth;
                 a.bounce = bo;
+                a.cull = (p->flags & EL_BATCH_NO_CULL) ? 0u : 1u;
                 const int cur = bo & 1, nxt = cur ^ 1;
                 uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
                 uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);

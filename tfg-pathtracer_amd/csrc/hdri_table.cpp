@@ -61,4 +61,39 @@ HdriSampler buildHdriSampler(const float* rgb, int w, int h) {
     return s;
 }
 
+std::vector<uint32_t> hdriBadBuckets(const HdriSampler& s, const float* rgb, int w, int h, float xTile, float yTile,
+                                     float xOffset, float yOffset) {
+    constexpr int kBuckets = 1 << 16;
+    std::vector<uint32_t> bits(kBuckets / 32, 0xFFFFFFFFu);
+    const int L = s.length;
+    if (L <= 0 || !(s.radianceSum > 0) || xTile != 1.f || yTile != 1.f || xOffset != 0.f || yOffset != 0.f)
+        return bits;
+    // bad[k]: the pdf of environment index k is infinite (row 0: sin 0) or zero
+    std::vector<uint8_t> bad(L);
+    for (int k = 0; k < L; k++) {
+        int qx, qy;
+        hdriPdfPixel(k, w, h, xTile, yTile, xOffset, yOffset, qx, qy);
+        const float* p = rgb + 3 * ((size_t)qy * w + qx);
+        bad[k] = qy == 0 || (p[0] + p[1] + p[2]) == 0.f;
+    }
+    std::fill(bits.begin(), bits.end(), 0u);
+    const float* cdf = s.cdf.data();
+    for (int b = 0; b < kBuckets; b++) {
+        const float rlo = (float)(b * 256) * (1.0f / 16777216.0f);
+        const float rhi = (float)(b * 256 + 255) * (1.0f / 16777216.0f);
+        // intervals j = max{ j < L : cdf[j] <= r } over the bucket
+        int jlo = (int)(std::upper_bound(cdf, cdf + L, rlo) - cdf) - 1;
+        int jhi = (int)(std::upper_bound(cdf, cdf + L, rhi) - cdf) - 1;
+        jlo = std::max(0, jlo);
+        jhi = std::max(jlo, jhi);
+        bool any = false;
+        for (int j = jlo; j <= jhi && !any; j++) {
+            const int a = s.map2[2 * (size_t)j], t = s.map2[2 * (size_t)j + 1];
+            any = (a >= 0 && a < L && bad[a]) || (t >= 0 && t < L && bad[t]) || a < 0 || a >= L || t < 0 || t >= L;
+        }
+        if (any) bits[b >> 5] |= 1u << (b & 31);
+    }
+    return bits;
+}
+
 }  // namespace eleven

@@ -37,6 +37,32 @@ int hdriSearchIndex(int j, int length);
 // The reference loop itself (HDRI.hpp:130-142), used for the tie table.
 int hdriBinarySearch(const float* cdf, float value, int length);
 
+// Pixel (qx, qy) at which HDRI::pdf (HDRI.hpp:145-152) is evaluated for the
+// environment sample of index k: inverseTransformUV (Texture.hpp:177-193) of
+// k's pixel corner, as the renderer's envSample computes it.
+EL_HD inline void hdriPdfPixel(int k, int w, int h, float xTile, float yTile, float xOffset, float yOffset, int& qx,
+                               int& qy) {
+    int x = k % w, y = k / w;
+    float nu = x / (float)w, nv = y / (float)h;
+    int ix = (int)(nu * w), iy = (int)(nv * h);
+    ix = (int)(xTile * (ix - xOffset * w)) % w;
+    iy = (int)(yTile * (iy - yOffset * h)) % h;
+    float iu = (float)ix / (float)w, iv = (float)iy / (float)h;
+    iu += -(float)(iu > 1) + -(float)(iu < 0);
+    iv += -(float)(iv > 1) + -(float)(iv < 0);
+    qx = (int)(iu * w);
+    qy = (int)(iv * h);
+}
+
+// Buckets of the environment random number whose sample can have a
+// non-finite or zero pdf: bit b (of 2^16) covers r = v * 2^-24 for
+// v in [256 b, 256 b + 255] (v = Philox word >> 8), set if any environment
+// index the search can return for such an r lands on the pole row (pdf
+// infinite) or on a black pixel (pdf zero).  Conservative: every bucket is
+// set when the texture uses tiling or offsets, or is black overall.
+std::vector<uint32_t> hdriBadBuckets(const struct HdriSampler& s, const float* rgb, int w, int h, float xTile,
+                                     float yTile, float xOffset, float yOffset);
+
 struct HdriSampler {
     std::vector<float> cdf;       // L + 1 entries, HDRI::generateCDF layout
     std::vector<int32_t> guide;   // G + 1 entries

@@ -281,6 +281,37 @@ __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, 
 }
 
 __device__ __forceinline__ bool anyNan(v3 a) { return isnan(a.x) || isnan(a.y) || isnan(a.z); }
+
+// A path whose throughput is exactly zero adds nothing more to its pixel --
+// unless a later vertex makes its radiance NaN, which rejects the whole sample
+// (kernel.cu:449).  With zero throughput the only such term (outside events
+// of measure zero) is the HDRI one: 0 * f * env / pdf is NaN when the sampled
+// environment pixel has an infinite pdf (pole row) or a zero one, and which
+// pixel a bounce samples depends on its random number alone.  True if no
+// remaining bounce's number falls in a bucket that can reach such a pixel
+// (s.envBad, conservative): stopping the path then leaves the estimate and
+// the accepted-sample count exactly as if it were traced.
+__device__ __forceinline__ bool quietDeath(const DevScene& s, const PathRng& rng, int fromBounce, int maxDepth) {
+    for (int b = fromBounce; b < maxDepth; b++) {
+        const uint32_t bucket = rng.block(2 + b).w >> 16;
+        if ((s.envBad[bucket >> 5] >> (bucket & 31)) & 1u) return false;
+    }
+    return true;
+}
+
+// Bit b set: bounce b's environment random number falls in a bucket that can
+// reach a pdf-0 / pdf-inf pixel (see quietDeath).  A path that dies at
+// bounce k stops quietly iff (mask >> (k + 1)) == 0.
+__device__ __forceinline__ uint32_t nanProneBounces(const DevScene& s, const PathRng& rng, int maxDepth) {
+    uint32_t m = 0;
+    for (int b = 0; b < maxDepth && b < 32; b++) {
+        const uint32_t bucket = rng.block(2 + b).w >> 16;
+        m |= ((s.envBad[bucket >> 5] >> (bucket & 31)) & 1u) << b;
+    }
+    return m;
+}
+
+__device__ __forceinline__ bool isZero(v3 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
 __device__ __forceinline__ v3 clamp3(v3 a) {
     return mk(clampf(a.x, 0.f, 10.f), clampf(a.y, 0.f, 10.f), clampf(a.z, 0.f, 10.f));
 }
@@ -343,6 +374,10 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
         T = vo.newT;
         o = vo.nextO;
         d = vo.nextD;
+        if (a.cull && isZero(T) && quietDeath(s, rng, i + 1, (int)a.maxDepth)) {
+            i++;
+            break;
+        }
     }
     verts += (uint32_t)i;
     return L;
@@ -575,12 +610,15 @@ EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
             q.pathAov[3 * (uint64_t)slot + 2] = make_float4(0, 0, 0, 0);
         }
         v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+        uint32_t quiet = 0xFFFFFFFFu;
         if (valid) {
             PathRng rng{a.k0, a.k1, idx, sample, a.scene};
             uint4 c = rng.block(0);
             uint32_t c5 = rng.block(1).x;
             cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
+            if (a.cull) quiet = nanProneBounces(s, rng, (int)a.maxDepth);
         }
+        q.pathQuiet[slot] = quiet;
         float4* Q = q.ray[0] + 2 * (uint64_t)slot;
         Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
         Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(a.scene));
@@ -711,7 +749,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                 // A shadow ray whose contribution is exactly zero cannot change L.
                 wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
                 wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
-                cont = !last;
+                cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
             }
         }
         const bool want[3] = {cont, wantH, wantP};
@@ -836,7 +874,7 @@ __device__ __forceinline__ uint32_t findSeg(const BatchSeg* segs, uint32_t nseg,
 // Camera rays of a batch generation: slot -> (scene, sample, pixel) through
 // the segment table; the scene id rides in the ray's d.w.
 EL_KERNEL(1) void k_bt_camera(ManyScenes sc, const BatchSeg* __restrict__ segs, uint32_t nseg, uint32_t genCount,
-                              WfState q) {
+                              uint32_t maxDepth, uint32_t cull, WfState q) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < genCount; slot += stride) {
         const BatchSeg g = segs[findSeg(segs, nseg, slot, false)];
@@ -852,12 +890,15 @@ EL_KERNEL(1) void k_bt_camera(ManyScenes sc, const BatchSeg* __restrict__ segs, 
         q.pathPix[slot] = idx;
         q.pathSample[slot] = sample;
         v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+        uint32_t quiet = 0xFFFFFFFFu;
         if (valid) {
             PathRng rng{s.seedLo, s.seedHi, idx, sample, g.scene};
             uint4 c = rng.block(0);
             uint32_t c5 = rng.block(1).x;
             cameraRay(s.cam, (int)x, (int)y, u01(c.x), u01(c.y), u01(c.z), u01(c.w), u01(c5), o, d);
+            if (cull) quiet = nanProneBounces(s, rng, (int)maxDepth);
         }
+        q.pathQuiet[slot] = quiet;
         float4* Q = q.ray[0] + 2 * (uint64_t)slot;
         Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
         Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(g.scene));
@@ -1022,10 +1063,10 @@ hipError_t launchWfFinalize(const LaunchCfg& c, const DevScene& s, const RenderA
 
 // ---- batch launchers
 hipError_t launchBtCamera(const LaunchCfg& c, const DevScene* scenes, const BatchSeg* segs, uint32_t nseg,
-                          uint32_t genCount, const WfState& q) {
+                          uint32_t genCount, uint32_t maxDepth, bool cull, const WfState& q) {
     uint32_t grid = gridFor(genCount, c.shadeBlock, c.maxThreads / c.shadeBlock);
     hipLaunchKernelGGL(k_bt_camera, dim3(grid), dim3(c.shadeBlock), 0, c.stream, ManyScenes{scenes}, segs, nseg,
-                       genCount, q);
+                       genCount, maxDepth, cull ? 1u : 0u, q);
     return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ struct RenderArgs {
     uint64_t genStart;      // wavefront: first work item of the generation
     uint32_t genCount;      // wavefront: work items in the generation
     uint32_t npad;          // wavefront: padded pixel count (8x8 tiles)
+    uint32_t cull;          // stop zero-throughput paths that cannot turn NaN
 };
 
 struct WfState {
@@ -52,6 +53,7 @@ struct WfState {
     float4* pathL;          // radiance per path slot, w = live flag
     uint32_t* pathPix;      // output pixel index per path slot
     uint32_t* pathSample;   // sample index per path slot
+    uint32_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b)
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
     uint32_t* counters;     // kQCounters: per-XCD segment counts of the four queues
     uint32_t segSize;       // entries per XCD segment of a compacted queue
@@ -89,7 +91,7 @@ hipError_t launchAdjoint(const LaunchCfg& c, const dev::DevScene& s, const Rende
 // Batch (many scenes, scene id per queue entry): scenes = device array of
 // DevScene; segs = the generation's segment table (device).
 hipError_t launchBtCamera(const LaunchCfg& c, const dev::DevScene* scenes, const BatchSeg* segs, uint32_t nseg,
-                          uint32_t genCount, const WfState& q);
+                          uint32_t genCount, uint32_t maxDepth, bool cull, const WfState& q);
 hipError_t launchBtExtend(const LaunchCfg& c, const dev::DevScene* scenes, int width, const WfState& q,
                           const float4* rayIn, const uint32_t* countIn, uint32_t* resetNext, uint32_t maxCount,
                           unsigned long long* counters);

@@ -20,7 +20,7 @@ LIB_PATH = PKG_DIR / "libeleven.so"
 
 EL_OK, EL_ERR_ARG, EL_ERR_IO, EL_ERR_HIP, EL_ERR_STATE, EL_ERR_RANGE, EL_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
 MODE_PLAIN, MODE_WAVEFRONT = 0, 1
-FLAG_AOV, FLAG_COUNT_TRAVERSAL = 1, 2
+FLAG_AOV, FLAG_COUNT_TRAVERSAL, FLAG_NO_CULL = 1, 2, 4
 PASS_BEAUTY, PASS_DENOISE, PASS_NORMAL, PASS_TANGENT, PASS_BITANGENT = 0, 1, 2, 3, 4
 
 
@@ -452,7 +452,7 @@ class Context:
         return ms.value
 
 
-BATCH_GRAPH = 1
+BATCH_GRAPH, BATCH_NO_CULL = 1, 4
 
 
 class Batch:
@@ -476,8 +476,9 @@ class Batch:
         self.scenes.append(scene)
         return idx.value
 
-    def render(self, depth=5, block=0, queue_cap=0, bvh_width=0, refill=0, graph=False) -> el_stats:
-        p = el_batch_params(depth, block, queue_cap, bvh_width, refill, BATCH_GRAPH if graph else 0)
+    def render(self, depth=5, block=0, queue_cap=0, bvh_width=0, refill=0, graph=False, cull=True) -> el_stats:
+        flags = (BATCH_GRAPH if graph else 0) | (0 if cull else BATCH_NO_CULL)
+        p = el_batch_params(depth, block, queue_cap, bvh_width, refill, flags)
         self._check(lib().el_batch_render(self._h, C.byref(p)))
         return self.stats()
 
