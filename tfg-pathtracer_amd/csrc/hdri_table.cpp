@@ -50,13 +50,23 @@ HdriSampler buildHdriSampler(const float* rgb, int w, int h) {
     while ((1 << k) < L) k++;
     s.log2G = k;
     const int G = 1 << k;
-    s.guide.resize((size_t)G + 1);
-    // guide[c] = max{ j < L : cdf[j] <= c / G }; c / G is exact in FP32.
+    s.guide.assign(2 * ((size_t)G + 1), -1);
+    // guide[2c] = max{ j < L : cdf[j] <= c / G }; c / G is exact in FP32.
     int j = 0;
     for (int c = 0; c <= G; c++) {
         float edge = (float)c / (float)G;
         while (j + 1 < L && s.cdf[j + 1] <= edge) j++;
-        s.guide[c] = j;
+        s.guide[2 * (size_t)c] = j;
+    }
+    // Cell c = [c/G, (c+1)/G) holds one result when every r in it lies in
+    // the interval j = guide[2c] strictly above cdf[j] (or the tie result is
+    // the same): no later boundary below (c+1)/G and cdf[j] < c/G.
+    for (int c = 0; c < G; c++) {
+        const int jc = s.guide[2 * (size_t)c];
+        const float edge = (float)c / (float)G;
+        const bool oneInterval = jc + 1 >= L || s.cdf[jc + 1] >= (float)(c + 1) / (float)G;
+        const bool noTie = s.cdf[jc] < edge || s.map2[2 * (size_t)jc] == s.map2[2 * (size_t)jc + 1];
+        if (oneInterval && noTie) s.guide[2 * (size_t)c + 1] = s.map2[2 * (size_t)jc];
     }
     return s;
 }

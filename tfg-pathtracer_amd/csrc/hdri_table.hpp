@@ -65,7 +65,10 @@ std::vector<uint32_t> hdriBadBuckets(const struct HdriSampler& s, const float* r
 
 struct HdriSampler {
     std::vector<float> cdf;       // L + 1 entries, HDRI::generateCDF layout
-    std::vector<int32_t> guide;   // G + 1 entries
+    std::vector<int32_t> guide;   // G + 1 pairs: [2c] = interval j(c / G),
+                                  // [2c + 1] = the result for every r of cell c
+                                  // when it is one value (no CDF boundary in
+                                  // the cell), else -1
     std::vector<int32_t> map2;    // [2j] = open[j], [2j+1] = tie[j]
     float radianceSum = 0;
     int32_t length = 0;           // L = W * H
@@ -78,9 +81,13 @@ struct HdriSampler {
 HdriSampler buildHdriSampler(const float* rgb, int width, int height);
 
 // The lookup shared by host and device: binarySearch(cdf, r, L) for r in [0, 1).
+// Samples concentrate on heavy CDF intervals, which span many guide cells
+// with no boundary inside: those cells answer from the guide entry alone.
 EL_HD inline int hdriLookup(const float* cdf, const int32_t* guide, const int32_t* map2, int log2G, float r) {
     int c = (int)(r * (float)(1 << log2G));
-    int lo = guide[c], hi = guide[c + 1];
+    const int direct = guide[2 * c + 1];
+    if (direct >= 0) return direct;
+    int lo = guide[2 * c], hi = guide[2 * c + 2];
     while (lo < hi) {
         int mid = (lo + hi + 1) >> 1;
         if (cdf[mid] <= r)

@@ -129,7 +129,7 @@ struct DevScene {
     const float* texels;           // pool of all textures (RGB FP32)
     const float* lights;           // 6 floats per light: position, radiance
     const float* envCdf;           // HDRI CDF (L + 1)
-    const int32_t* envGuide;       // guide table (2^envLog2G + 1)
+    const int32_t* envGuide;       // guide table (2^envLog2G + 1 pairs)
     const int32_t* envMap;         // per interval: open-interval and tie results
     const uint32_t* envBad;        // 2^16-bit map: random-number buckets whose HDRI pdf may be 0 or inf
     DevTex env;                    // env texture (texels in the pool)
