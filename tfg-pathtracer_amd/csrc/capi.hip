@@ -88,7 +88,7 @@ struct el_ctx {
     std::vector<dev::DevMaterial> mats;
     std::vector<dev::DevTex> texs;
 
-    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad;
+    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad, dEnvTab;
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
     DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters;
     size_t wfCap = 0, wfQcap = 0;
@@ -185,6 +185,32 @@ const char* el_last_error(const el_ctx* c) { return c ? c->err.c_str() : "null c
 }  // extern "C"
 
 // Pack and upload the wide BVH of the current width.
+// Material values as the kernels read them (texture ids are set by the
+// caller); for untextured roughness and metallic also generateHitData's
+// powf(x, 2.2f) (kernel.cu:113-114), once per material.
+static void setMaterialValues(dev::DevMaterial& d, const el_material& m) {
+    d.albedo[0] = m.albedo.x;
+    d.albedo[1] = m.albedo.y;
+    d.albedo[2] = m.albedo.z;
+    d.emission[0] = m.emission.x;
+    d.emission[1] = m.emission.y;
+    d.emission[2] = m.emission.z;
+    d.roughness = m.roughness;
+    d.metallic = m.metallic;
+    d.roughnessEff = powf(m.roughness, 2.2f);
+    d.metallicEff = powf(m.metallic, 2.2f);
+    d.clearcoatGloss = m.clearcoat_gloss;
+    d.clearcoat = m.clearcoat;
+    d.anisotropic = m.anisotropic;
+    d.eta = m.eta;
+    d.transmission = m.transmission;
+    d.specular = m.specular;
+    d.specularTint = m.specular_tint;
+    d.sheenTint = m.sheen_tint;
+    d.subsurface = m.subsurface;
+    d.sheen = m.sheen;
+}
+
 static int uploadBvh(el_ctx* c, int width) {
     try {
         c->wide = collapseWide(c->bvh2, c->verts.data(), (int)c->ntri, width);
@@ -297,24 +323,7 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
         d.roughnessTex = texOk(m.roughness_tex);
         d.metallicTex = texOk(m.metallic_tex);
         d.normalTex = texOk(m.normal_tex);
-        d.albedo[0] = m.albedo.x;
-        d.albedo[1] = m.albedo.y;
-        d.albedo[2] = m.albedo.z;
-        d.emission[0] = m.emission.x;
-        d.emission[1] = m.emission.y;
-        d.emission[2] = m.emission.z;
-        d.roughness = m.roughness;
-        d.metallic = m.metallic;
-        d.clearcoatGloss = m.clearcoat_gloss;
-        d.clearcoat = m.clearcoat;
-        d.anisotropic = m.anisotropic;
-        d.eta = m.eta;
-        d.transmission = m.transmission;
-        d.specular = m.specular;
-        d.specularTint = m.specular_tint;
-        d.sheenTint = m.sheen_tint;
-        d.subsurface = m.subsurface;
-        d.sheen = m.sheen;
+        setMaterialValues(d, m);
     }
     EL_HIP(c, upload(c->dMats, c->mats.data(), c->mats.size() * sizeof(dev::DevMaterial), c->stream));
 
@@ -361,6 +370,8 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     s.envBad = c->dEnvBad.as<uint32_t>();
     s.env = env;
     s.envRadianceSum = hs.radianceSum;
+    EL_HIP(c, c->dEnvTab.ensure((size_t)h.width * h.height * 3 * sizeof(float4)));
+    s.envTab = c->dEnvTab.as<float4>();
     s.envLog2G = hs.log2G;
     s.nLights = (int32_t)v->light_count;
     s.nTris = (int32_t)v->tri_count;
@@ -387,6 +398,7 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     dc.pos[1] = v->camera.position.y;
     dc.pos[2] = v->camera.position.z;
     dc.bokeh = v->camera.bokeh;
+    EL_HIP(c, launchEnvTable(c->stream, s, c->dEnvTab.as<float4>()));
     rc = clearAccum(c);
     if (rc) return rc;
     c->hasScene = true;
@@ -723,25 +735,7 @@ int el_set_materials(el_ctx* c, const el_material* m, uint32_t count) {
     if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
     if (count != c->mats.size()) return fail(c, EL_ERR_ARG, "material count differs from the uploaded scene");
     for (uint32_t i = 0; i < count; i++) {
-        dev::DevMaterial& d = c->mats[i];
-        d.albedo[0] = m[i].albedo.x;
-        d.albedo[1] = m[i].albedo.y;
-        d.albedo[2] = m[i].albedo.z;
-        d.emission[0] = m[i].emission.x;
-        d.emission[1] = m[i].emission.y;
-        d.emission[2] = m[i].emission.z;
-        d.roughness = m[i].roughness;
-        d.metallic = m[i].metallic;
-        d.clearcoatGloss = m[i].clearcoat_gloss;
-        d.clearcoat = m[i].clearcoat;
-        d.anisotropic = m[i].anisotropic;
-        d.eta = m[i].eta;
-        d.transmission = m[i].transmission;
-        d.specular = m[i].specular;
-        d.specularTint = m[i].specular_tint;
-        d.sheenTint = m[i].sheen_tint;
-        d.subsurface = m[i].subsurface;
-        d.sheen = m[i].sheen;
+        setMaterialValues(c->mats[i], m[i]);
     }
     EL_HIP(c, hipSetDevice(c->device));
     EL_HIP(c, hipMemcpyAsync(c->dMats.p, c->mats.data(), count * sizeof(dev::DevMaterial), hipMemcpyHostToDevice,

@@ -227,18 +227,18 @@ __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, 
                                                  const HitData& hd, const uint32_t* r) {
     VertexOut o;
     v3 bdir = disneySample(d, hd, u01(r[0]), u01(r[1]), u01(r[2]));
-    v3 f = disneyEval(d, hd, bdir);
-    float bpdf = disneyPdf(d, hd, bdir);
+    float bpdf;
+    v3 f = disneyEvalPdf(d, hd, bdir, bpdf);
     float r1 = u01(r[3]);
 
     EnvSample es = envSample(s, r1);
-    v3 H = disneyEval(d, hd, es.dir) * fabsf(dot(es.dir, hd.normal)) * es.value / es.pdf;
+    v3 H = disneyEval(d, hd, es.dir) * fabsf(dot(es.dir, hd.normal)) * es.value * frcp(es.pdf);
     float hpdf = es.pdf;
     o.hOrig = sh.position + es.dir * 0.001f;
-    o.hDir = normalized(es.dir);
+    o.hDir = es.shadowDir;
 
     float ppdf = 0.f;
-    v3 P = mk(0, 0, 0);
+    v3 P = mk(0, 0, 0), ldir = mk(0, 0, 1), lrad = mk(0, 0, 0);
     o.hasP = s.nLights > 0;
     if (o.hasP) {
         ppdf = ((float)s.nLights) / (float)(2.0 * EL_PI);
@@ -250,21 +250,21 @@ __device__ __forceinline__ VertexOut shadeVertex(const DevScene& s, v3 d, v3 T, 
         o.pOrig = sh.position + nd * 0.001f;
         o.pDir = normalized(nd);
         o.pTmax = dist - 0.001f;
-        P = (lr / (dist * dist)) * disneyEval(d, hd, nd) * fabsf(dot(nd, hd.normal)) / ppdf;
-        o.ldir = nd;
-        o.lrad = lr / (dist * dist);
-    } else {
-        o.ldir = mk(0, 0, 1);
-        o.lrad = mk(0, 0, 0);
+        lrad = lr * frcp(dist * dist);
+        P = lrad * disneyEval(d, hd, nd) * fabsf(dot(nd, hd.normal)) * frcp(ppdf);
+        ldir = nd;
     }
+    o.ldir = ldir;
+    o.lrad = lrad;
     v3 fc = f * fabsf(dot(bdir, hd.normal));
-    v3 B = hd.emission * fc / bpdf;
-    float sum = hpdf + ppdf + bpdf;
-    float w1 = hpdf / sum, w2 = ppdf / sum, w3 = bpdf / sum;
+    float rb = frcp(bpdf);
+    v3 B = hd.emission * fc * rb;
+    float rs = frcp(hpdf + ppdf + bpdf);
+    float w1 = hpdf * rs, w2 = ppdf * rs, w3 = bpdf * rs;
     o.base = T * (w1 * mk(0, 0, 0) + w2 * mk(0, 0, 0) + w3 * B);
     o.contribH = T * (w1 * H);
     o.contribP = T * (w2 * P);
-    o.newT = T * (fc / bpdf);
+    o.newT = T * (fc * rb);
     o.nextO = sh.position + bdir * 0.001f;
     o.nextD = normalized(bdir);
     o.bdir = bdir;
@@ -934,6 +934,16 @@ EL_KERNEL(1) void k_bt_finalize(ManyScenes sc, const BatchSeg* __restrict__ segs
     blockFlush<1>(counters, slots, vals);
 }
 
+// envTab[3k .. 3k+2] = envSampleAt(k) for every environment pixel k < n.
+__global__ void __launch_bounds__(256) k_env_table(DevScene s, float4* tab, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const EnvSample e = envSampleAt(s, (int)k);
+    tab[3 * (uint64_t)k] = make_float4(e.dir.x, e.dir.y, e.dir.z, e.pdf);
+    tab[3 * (uint64_t)k + 1] = make_float4(e.value.x, e.value.y, e.value.z, 0.f);
+    tab[3 * (uint64_t)k + 2] = make_float4(e.shadowDir.x, e.shadowDir.y, e.shadowDir.z, 0.f);
+}
+
 }  // namespace dev
 
 // ------------------------------------------------------------ host launch --
@@ -998,6 +1008,12 @@ hipError_t launchAdjoint(const LaunchCfg& c, const DevScene& s, const RenderArgs
     else
         hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, gradTex,
                            dmat, dtex, c.gstack, c.maxThreads, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchEnvTable(hipStream_t stream, const DevScene& s, float4* tab) {
+    const uint32_t n = (uint32_t)s.env.w * (uint32_t)s.env.h;
+    hipLaunchKernelGGL(k_env_table, dim3((n + 255) / 256), dim3(256), 0, stream, s, tab, n);
     return hipGetLastError();
 }
 

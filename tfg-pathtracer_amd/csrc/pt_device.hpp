@@ -54,6 +54,18 @@ __device__ __forceinline__ v3 normalized(v3 a) {
     if (l == 0.f) return a;
     return a / l;
 }
+// Hardware reciprocal and square roots (v_rcp_f32, v_sqrt_f32, v_rsq_f32:
+// 1 ulp) for value arithmetic only -- BRDF and pdf values, MIS weights,
+// throughput.  Every quantity that decides a direction, an index or a branch
+// of the path (sampling, bases, ray and shadow directions, texel and HDRI
+// index math, intersection) keeps IEEE division and square root.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+// normalized() for value arithmetic: zero vectors pass through unchanged.
+__device__ __forceinline__ v3 fnormalized(v3 a) {
+    float l2 = dot(a, a);
+    return l2 == 0.f ? a : a * __builtin_amdgcn_rsqf(l2);
+}
 __device__ __forceinline__ float lerpf(float a, float b, float c) { return a + c * (b - a); }  // FAST_LERP
 __device__ __forceinline__ v3 lerp3(v3 a, v3 b, float c) {
     return mk(lerpf(a.x, b.x, c), lerpf(a.y, b.y, c), lerpf(a.z, b.z, c));
@@ -105,6 +117,8 @@ struct DevMaterial {
     float albedo[3], emission[3];
     float roughness, metallic, clearcoatGloss, clearcoat, anisotropic, eta, transmission, specular,
         specularTint, sheenTint, subsurface, sheen;
+    float roughnessEff, metallicEff;   // powf(roughness, 2.2f), powf(metallic, 2.2f) (kernel.cu:113-114)
+    float pad1[2];
 };
 
 struct DevCamera {
@@ -132,6 +146,7 @@ struct DevScene {
     const int32_t* envGuide;       // guide table (2^envLog2G + 1 pairs)
     const int32_t* envMap;         // per interval: open-interval and tie results
     const uint32_t* envBad;        // 2^16-bit map: random-number buckets whose HDRI pdf may be 0 or inf
+    const float4* envTab;          // 3 per HDRI pixel k: envSampleAt(k) (direction + pdf, value, shadow direction)
     DevTex env;                    // env texture (texels in the pool)
     float envRadianceSum;
     int32_t envLog2G;
@@ -235,47 +250,72 @@ __device__ __forceinline__ float schlickFresnel(float u) {
     float m2 = m * m;
     return m2 * m2 * m;
 }
+// The microfacet terms below are value arithmetic (frcp, fsqrt).
 __device__ __forceinline__ float GTR1(float NDotH, float a) {
     if (a >= 1.0f) return 1.0f / EL_PI;
     float a2 = a * a;
     float t = 1.0f + (a2 - 1.0f) * NDotH * NDotH;
-    return (a2 - 1.0f) / (EL_PI * logf(a2) * t);
+    return (a2 - 1.0f) * frcp(EL_PI * logf(a2) * t);
 }
-__device__ __forceinline__ float GTR2_aniso(float NDotH, float HDotX, float HDotY, float ax, float ay) {
-    float a = HDotX / ax, b = HDotY / ay;
+// 1 / (pi ax ay c^2) with rax = 1 / ax, ray = 1 / ay, piAxAy = pi * ax * ay.
+__device__ __forceinline__ float GTR2_aniso(float NDotH, float HDotX, float HDotY, float rax, float ray,
+                                            float piAxAy) {
+    float a = HDotX * rax, b = HDotY * ray;
     float c = a * a + b * b + NDotH * NDotH;
-    return 1.0f / (EL_PI * ax * ay * c * c);
+    return frcp(piAxAy * c * c);
 }
 __device__ __forceinline__ float smithG_GGX(float NDotV, float alphaG) {
     float a = alphaG * alphaG, b = NDotV * NDotV;
-    return 1.0f / (NDotV + sqrtf(a + b - a * b));
+    return frcp(NDotV + fsqrt(a + b - a * b));
 }
 __device__ __forceinline__ float smithG_GGX_aniso(float NDotV, float VDotX, float VDotY, float ax, float ay) {
     float a = VDotX * ax, b = VDotY * ay, c = NDotV;
-    return 1.0f / (NDotV + sqrtf(a * a + b * b + c * c));
+    return frcp(NDotV + fsqrt(a * a + b * b + c * c));
 }
 
-// Disney.hpp:108-147.  `dir` is the incoming ray direction (V = -dir).
-__device__ __forceinline__ float disneyPdf(v3 dir, const HitData& h, v3 L) {
+// Anisotropic GGX widths of Disney.hpp:128-130 / 213-215.
+struct GgxWidths {
+    float ax, ay, rax, ray, piAxAy;
+};
+__device__ __forceinline__ GgxWidths ggxWidths(const HitData& h) {
+    GgxWidths g;
+    float aspect = fsqrt(1.0f - h.anisotropic * 0.9f);
+    float ra = frcp(aspect);
+    g.ax = maxf(0.001f, h.roughness * ra);
+    g.ay = maxf(0.001f, h.roughness * aspect);
+    g.rax = frcp(g.ax);
+    g.ray = frcp(g.ay);
+    g.piAxAy = EL_PI * g.ax * g.ay;
+    return g;
+}
+
+// Disney.hpp:108-147 given the half vector H = normalize(L + V) and
+// Ds = GTR2_aniso(N.H, H.T, H.B).  `dir` is the incoming ray direction
+// (V = -dir).  With clearcoat == 0 the GTR1/GTR2 mix weight is exactly 1.
+__device__ __forceinline__ float disneyPdfH(const HitData& h, v3 L, v3 H, float NDotH, float Ds) {
     v3 N = h.normal;
-    v3 V = dir * -1.0f;
-    v3 H = normalized(L + V);
-    v3 T, B;
-    createBasis(h.normal, T, B);
-    float NDotH = fabsf(dot(N, H));
     if (dot(N, L) <= 0.0f) return 1.0f;
-    float clearcoatAlpha = lerpf(0.1f, 0.001f, h.clearcoatGloss);
     float diffuseRatio = 0.5f * (1.0f - h.metallic);
     float specularRatio = 1.0f - diffuseRatio;
-    float aspect = sqrtf(1.0f - h.anisotropic * 0.9f);
-    float ax = maxf(0.001f, h.roughness / aspect);
-    float ay = maxf(0.001f, h.roughness * aspect);
-    float pdfGTR2 = GTR2_aniso(NDotH, dot(H, T), dot(H, B), ax, ay) * NDotH;
-    float pdfGTR1 = GTR1(NDotH, clearcoatAlpha) * NDotH;
-    float ratio = 1.0f / (1.0f + h.clearcoat);
-    float pdfSpec = lerpf(pdfGTR1, pdfGTR2, ratio) / (4.0f * fabsf(dot(L, H)));
+    float pdfGTR2 = Ds * NDotH;
+    float mix = pdfGTR2;
+    if (h.clearcoat != 0.0f) {
+        float clearcoatAlpha = lerpf(0.1f, 0.001f, h.clearcoatGloss);
+        float pdfGTR1 = GTR1(NDotH, clearcoatAlpha) * NDotH;
+        mix = lerpf(pdfGTR1, pdfGTR2, frcp(1.0f + h.clearcoat));
+    }
+    float pdfSpec = mix * frcp(4.0f * fabsf(dot(L, H)));
     float pdfDiff = fabsf(dot(L, N)) * (1.0f / EL_PI);
     return diffuseRatio * pdfDiff + specularRatio * pdfSpec;
+}
+__device__ __forceinline__ float disneyPdf(v3 dir, const HitData& h, v3 L) {
+    v3 V = dir * -1.0f;
+    v3 H = fnormalized(L + V);
+    v3 T, B;
+    createBasis(h.normal, T, B);
+    float NDotH = fabsf(dot(h.normal, H));
+    GgxWidths g = ggxWidths(h);
+    return disneyPdfH(h, L, H, NDotH, GTR2_aniso(NDotH, dot(H, T), dot(H, B), g.rax, g.ray, g.piAxAy));
 }
 
 // Disney.hpp:150-177
@@ -295,21 +335,27 @@ __device__ __forceinline__ v3 disneySample(v3 dir, const HitData& h, float r1, f
     return v1 - 2 * dot(v1, H) * H;   // reflect (Vector.hpp:218-220)
 }
 
-// Disney.hpp:179-253
-__device__ __forceinline__ v3 disneyEval(v3 dir, const HitData& h, v3 L) {
+// Disney.hpp:179-253; with PDF also disneyPdf(dir, h, L) into `pdf`, which
+// shares the half vector and the GTR2 term.  The clearcoat lobe is skipped
+// when its weight is 0 (it is finite, so it adds exactly 0).
+template <bool PDF>
+__device__ __forceinline__ v3 disneyEvalT(v3 dir, const HitData& h, v3 L, float& pdf) {
     v3 V = dir * -1.0f;
     v3 T, B;
     createBasis(h.normal, T, B);
-    v3 H = normalized(L + V);
+    v3 H = fnormalized(L + V);
     float NDotL = fabsf(dot(h.normal, L));
     float NDotV = fabsf(dot(h.normal, V));
     float NDotH = fabsf(dot(h.normal, H));
     float LDotH = fabsf(dot(L, H));
+    GgxWidths g = ggxWidths(h);
+    float Ds = GTR2_aniso(NDotH, dot(H, T), dot(H, B), g.rax, g.ray, g.piAxAy);
+    if (PDF) pdf = disneyPdfH(h, L, H, NDotH, Ds);
     v3 brdf = mk(0, 0, 0);
     if (h.transmission < 1.0f && dot(h.normal, L) > 0.0f && dot(h.normal, V) > 0.0f) {
         v3 Cdlin = h.albedo;
         float Cdlum = 0.3f * Cdlin.x + 0.6f * Cdlin.y + 0.1f * Cdlin.z;
-        v3 Ctint = Cdlum > 0.0f ? Cdlin / Cdlum : mk(1, 1, 1);
+        v3 Ctint = Cdlum > 0.0f ? Cdlin * frcp(Cdlum) : mk(1, 1, 1);
         v3 Cspec0 = lerp3(h.specular * 0.08f * lerp3(mk(1, 1, 1), Ctint, h.specularTint), Cdlin, h.metallic);
         v3 Csheen = lerp3(mk(1, 1, 1), Ctint, h.sheenTint);
         float FL = schlickFresnel(NDotL), FV = schlickFresnel(NDotV);
@@ -317,23 +363,28 @@ __device__ __forceinline__ v3 disneyEval(v3 dir, const HitData& h, v3 L) {
         float Fd = lerpf(1.0f, Fd90, FL) * lerpf(1.0f, Fd90, FV);
         float Fss90 = LDotH * LDotH * h.roughness;
         float Fss = lerpf(1.0f, Fss90, FL) * lerpf(1.0f, Fss90, FV);
-        float ss = 1.25f * (Fss * (1.0f / (NDotL + NDotV) - 0.5f) + 0.5f);
-        float aspect = sqrtf(1.0f - h.anisotropic * 0.9f);
-        float ax = maxf(0.001f, h.roughness / aspect);
-        float ay = maxf(0.001f, h.roughness * aspect);
-        float Ds = GTR2_aniso(NDotH, dot(H, T), dot(H, B), ax, ay);
+        float ss = 1.25f * (Fss * (frcp(NDotL + NDotV) - 0.5f) + 0.5f);
         float FH = schlickFresnel(LDotH);
         v3 Fs = lerp3(Cspec0, mk(1, 1, 1), FH);
-        float Gs = smithG_GGX_aniso(NDotL, dot(L, T), dot(L, B), ax, ay);
-        Gs *= smithG_GGX_aniso(NDotV, dot(V, T), dot(V, B), ax, ay);
+        float Gs = smithG_GGX_aniso(NDotL, dot(L, T), dot(L, B), g.ax, g.ay);
+        Gs *= smithG_GGX_aniso(NDotV, dot(V, T), dot(V, B), g.ax, g.ay);
         v3 Fsheen = FH * h.sheen * Csheen;
-        float Dr = GTR1(NDotH, lerpf(0.1f, 0.001f, h.clearcoatGloss));
-        float Fr = lerpf(0.04f, 1.0f, FH);
-        float Gr = smithG_GGX(NDotL, 0.25f) * smithG_GGX(NDotV, 0.25f);
-        brdf = ((1.0f / EL_PI) * lerpf(Fd, ss, h.subsurface) * Cdlin + Fsheen) * (1.0f - h.metallic) +
-               Gs * Fs * Ds + mk(1, 1, 1) * (0.25f * h.clearcoat * Gr * Fr * Dr);
+        brdf = ((1.0f / EL_PI) * lerpf(Fd, ss, h.subsurface) * Cdlin + Fsheen) * (1.0f - h.metallic) + Gs * Fs * Ds;
+        if (h.clearcoat != 0.0f) {
+            float Dr = GTR1(NDotH, lerpf(0.1f, 0.001f, h.clearcoatGloss));
+            float Fr = lerpf(0.04f, 1.0f, FH);
+            float Gr = smithG_GGX(NDotL, 0.25f) * smithG_GGX(NDotV, 0.25f);
+            brdf = brdf + mk(1, 1, 1) * (0.25f * h.clearcoat * Gr * Fr * Dr);
+        }
     }
     return brdf;
+}
+__device__ __forceinline__ v3 disneyEval(v3 dir, const HitData& h, v3 L) {
+    float unused;
+    return disneyEvalT<false>(dir, h, L, unused);
+}
+__device__ __forceinline__ v3 disneyEvalPdf(v3 dir, const HitData& h, v3 L, float& pdf) {
+    return disneyEvalT<true>(dir, h, L, pdf);
 }
 
 // d disneyEval / d (albedo.x, albedo.y, albedo.z, roughness, metallic,
@@ -484,8 +535,12 @@ __device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& 
     HitData d;
     d.albedo = m.albedoTex < 0 ? ld3(m.albedo) : texFiltered(s.texels, s.texs[m.albedoTex], h.tu, h.tv);
     d.emission = m.emissionTex < 0 ? ld3(m.emission) : texFiltered(s.texels, s.texs[m.emissionTex], h.tu, h.tv);
-    d.roughness = m.roughnessTex < 0 ? m.roughness : texFiltered(s.texels, s.texs[m.roughnessTex], h.tu, h.tv).x;
-    d.metallic = m.metallicTex < 0 ? m.metallic : texFiltered(s.texels, s.texs[m.metallicTex], h.tu, h.tv).x;
+    // powf(x, 2.2f) of kernel.cu:113-114: for untextured parameters computed
+    // once per material at upload
+    d.roughness = m.roughnessTex < 0 ? m.roughnessEff
+                                     : powf(texFiltered(s.texels, s.texs[m.roughnessTex], h.tu, h.tv).x, 2.2f);
+    d.metallic = m.metallicTex < 0 ? m.metallicEff
+                                   : powf(texFiltered(s.texels, s.texs[m.metallicTex], h.tu, h.tv).x, 2.2f);
     if (m.normalTex < 0) {
         d.normal = h.normal;
     } else {
@@ -493,8 +548,6 @@ __device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& 
         v3 ln = c * 2.0f - mk(1, 1, 1);
         d.normal = normalized(ln.x * h.tangent - ln.y * h.bitangent + ln.z * h.normal);
     }
-    d.roughness = powf(d.roughness, 2.2f);
-    d.metallic = powf(d.metallic, 2.2f);
     d.clearcoatGloss = m.clearcoatGloss;
     d.clearcoat = m.clearcoat;
     d.anisotropic = m.anisotropic;
@@ -521,10 +574,11 @@ __device__ __forceinline__ v3 envLookup(const DevScene& s, v3 d) {
 struct EnvSample {
     v3 dir, value;
     float pdf;
+    v3 shadowDir;   // normalized(dir), the shadow ray's direction
 };
-__device__ __forceinline__ EnvSample envSample(const DevScene& s, float r1) {
-    // HDRI::sample(r1): exactly binarySearch(cdf, r1, W*H)
-    int k = hdriLookup(s.envCdf, s.envGuide, s.envMap, s.envLog2G, r1);
+// Everything the NEE sample derives from the environment pixel k alone;
+// tabulated per pixel at upload (envTab) by this same code.
+__device__ __forceinline__ EnvSample envSampleAt(const DevScene& s, int k) {
     const DevTex& t = s.env;
     int x = k % t.w, y = k / t.w;
     float nu = x / (float)t.w, nv = y / (float)t.h;
@@ -547,6 +601,18 @@ __device__ __forceinline__ EnvSample envSample(const DevScene& s, float r1) {
     v3 dv = texel(s.texels, t, qx, qy);
     float th = (((float)qy / (float)t.h)) * EL_PI;
     e.pdf = ((dv.x + dv.y + dv.z) / s.envRadianceSum) * (float)(t.w * t.h) / (2.0f * EL_PI * sinf(th));
+    e.shadowDir = normalized(e.dir);
+    return e;
+}
+__device__ __forceinline__ EnvSample envSample(const DevScene& s, float r1) {
+    // HDRI::sample(r1): exactly binarySearch(cdf, r1, W*H)
+    const int k = hdriLookup(s.envCdf, s.envGuide, s.envMap, s.envLog2G, r1);
+    const float4 a = s.envTab[3 * k], b = s.envTab[3 * k + 1], c = s.envTab[3 * k + 2];
+    EnvSample e;
+    e.dir = mk(a.x, a.y, a.z);
+    e.pdf = a.w;
+    e.value = mk(b.x, b.y, b.z);
+    e.shadowDir = mk(c.x, c.y, c.z);
     return e;
 }
 
