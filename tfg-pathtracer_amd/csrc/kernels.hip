@@ -690,71 +690,110 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
     blockFlush<4>(counters, slots, vals);
 }
 
+// Escaped rays (a texture lookup) and hits (full shading) arrive mixed in
+// the queue.  Each block classifies a chunk of blockDim entries, adds the
+// environment term of its escaped rays at once and pushes its hits onto an
+// LDS pool; hits are shaded blockDim at a time from the top of the pool, so
+// the long shading code runs on full waves.  Results do not depend on the
+// order: every path has one entry per queue.
 template <class P>
 EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const float4* __restrict__ rayIn,
                            const uint32_t* countIn, float4* __restrict__ rayOut, uint32_t* countOut,
                            unsigned long long* counters) {
+    __shared__ uint32_t s_pool[2 * kMaxBlock];
+    __shared__ uint32_t s_wcnt[kMaxWaves];
     QueueMap qm;
     qm.load(countIn);
     const uint32_t n = qm.total();
-    uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t stride = gridDim.x * blockDim.x;
     uint32_t rays = 0, verts = 0, overflow = 0;
     const int bounce = (int)a.bounce;
     const bool last = bounce + 1 >= (int)a.maxDepth;
     uint32_t* const outCnt[3] = {countOut, q.counters + kQShadowH, q.counters + kQShadowP};
-    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += stride) {
-        uint32_t i = b + threadIdx.x;
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = __lane_id();
+    const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    uint32_t pending = 0;   // hits in the pool (block-uniform)
+    for (uint32_t b = blockIdx.x * blockDim.x;; b += stride) {
+        const bool more = b < n;
+        if (more) {
+            const uint32_t i = b + threadIdx.x;
+            uint32_t e = 0;
+            bool hit = false;
+            if (i < n) {
+                e = (uint32_t)qm.entry(i, q.segSize);
+                const int tri = __float_as_int(q.hits[e].x);
+                if (tri == -1) {
+                    const float4 A = rayIn[2 * (uint64_t)e], B = rayIn[2 * (uint64_t)e + 1];
+                    const uint32_t slot = __float_as_uint(A.w);
+                    const DevScene& s = sc.get(__float_as_uint(B.w));
+                    const float4 Tf = q.pathT[slot];
+                    const v3 ev = envLookup(s, mk(B.x, B.y, B.z)) * mk(Tf.x, Tf.y, Tf.z);
+                    const float4 L = q.pathL[slot];
+                    q.pathL[slot] = make_float4(L.x + ev.x, L.y + ev.y, L.z + ev.z, L.w);
+                }
+                hit = tri >= 0;
+            }
+            const unsigned long long m = __ballot(hit);
+            if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
+            __syncthreads();
+            uint32_t off = pending, total = 0;
+            for (int w = 0; w < nw; w++) {
+                const uint32_t c = s_wcnt[w];
+                off += w < wave ? c : 0u;
+                total += c;
+            }
+            if (hit) s_pool[off + (uint32_t)__popcll(m & lt)] = e;
+            pending += total;
+            __syncthreads();   // pool entries visible, s_wcnt free
+            if (pending < blockDim.x) continue;
+        } else if (pending == 0) {
+            break;
+        }
+        const uint32_t take = pending < blockDim.x ? pending : blockDim.x;
+        const bool has = threadIdx.x < take;
+        const uint32_t e = has ? s_pool[pending - take + threadIdx.x] : 0u;
+        pending -= take;
         bool cont = false, wantH = false, wantP = false;
         VertexOut vo;
         uint32_t slot = 0, sid = 0;
-        if (i < n) {
-            uint64_t e = qm.entry(i, q.segSize);
-            float4 A = rayIn[2 * e], B = rayIn[2 * e + 1];
-            float4 hr = q.hits[e];
+        if (has) {
+            const float4 A = rayIn[2 * (uint64_t)e], B = rayIn[2 * (uint64_t)e + 1];
+            const float4 hr = q.hits[e];
             slot = __float_as_uint(A.w);
             sid = __float_as_uint(B.w);
             const DevScene& s = sc.get(sid);
-            int tri = __float_as_int(hr.x);
-            if (tri == -1) {
-                v3 d = mk(B.x, B.y, B.z);
-                float4 Tf = q.pathT[slot];
-                v3 ev = envLookup(s, d) * mk(Tf.x, Tf.y, Tf.z);
-                float4 L = q.pathL[slot];
-                q.pathL[slot] = make_float4(L.x + ev.x, L.y + ev.y, L.z + ev.z, L.w);
-            } else if (tri >= 0) {
-                v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
-                float4 Tf = q.pathT[slot];
-                SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
-                HitData hd = hitData(s, sh);
-                uint32_t k0 = a.k0, k1 = a.k1;
-                if (P::kMany) {
-                    k0 = s.seedLo;
-                    k1 = s.seedHi;
-                }
-                PathRng rng{k0, k1, q.pathPix[slot], q.pathSample[slot], sid};
-                uint4 rb = rng.block(2 + bounce);
-                uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
-                vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
-                float4 L = q.pathL[slot];
-                q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
-                q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
-                if (bounce == 0 && q.pathAov) {
-                    q.pathAov[3 * (uint64_t)slot + 0] = make_float4(sh.normal.x, sh.normal.y, sh.normal.z, 0);
-                    q.pathAov[3 * (uint64_t)slot + 1] = make_float4(sh.tangent.x, sh.tangent.y, sh.tangent.z, 0);
-                    q.pathAov[3 * (uint64_t)slot + 2] =
-                        make_float4(sh.bitangent.x, sh.bitangent.y, sh.bitangent.z, 0);
-                }
-                verts++;
-                rays += 1 + (vo.hasP ? 1 : 0);
-                // A shadow ray whose contribution is exactly zero cannot change L.
-                wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
-                wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
-                cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
+            const int tri = __float_as_int(hr.x);
+            v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
+            float4 Tf = q.pathT[slot];
+            SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
+            HitData hd = hitData(s, sh);
+            uint32_t k0 = a.k0, k1 = a.k1;
+            if (P::kMany) {
+                k0 = s.seedLo;
+                k1 = s.seedHi;
             }
+            PathRng rng{k0, k1, q.pathPix[slot], q.pathSample[slot], sid};
+            uint4 rb = rng.block(2 + bounce);
+            uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
+            vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
+            float4 L = q.pathL[slot];
+            q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
+            q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
+            if (bounce == 0 && q.pathAov) {
+                q.pathAov[3 * (uint64_t)slot + 0] = make_float4(sh.normal.x, sh.normal.y, sh.normal.z, 0);
+                q.pathAov[3 * (uint64_t)slot + 1] = make_float4(sh.tangent.x, sh.tangent.y, sh.tangent.z, 0);
+                q.pathAov[3 * (uint64_t)slot + 2] = make_float4(sh.bitangent.x, sh.bitangent.y, sh.bitangent.z, 0);
+            }
+            verts++;
+            rays += 1 + (vo.hasP ? 1 : 0);
+            // A shadow ray whose contribution is exactly zero cannot change L.
+            wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
+            wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
+            cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
         }
         const bool want[3] = {cont, wantH, wantP};
         uint64_t at[3];
-        blockAlloc3(want, outCnt, q.segSize, at, overflow);
+        blockAlloc3(want, outCnt, q.segSize, at, overflow);   // ends with a barrier: the pool may be refilled
         if (at[0] != ~0ull) {
             float4* Q = rayOut + 2 * at[0];
             Q[0] = make_float4(vo.nextO.x, vo.nextO.y, vo.nextO.z, __uint_as_float(slot));
