@@ -282,8 +282,9 @@ int  el_batch_add(el_batch* batch, const el_scene_view* view, uint32_t spp,
 /* Renders every scene's spp from cleared accumulations (synchronous). */
 int  el_batch_render(el_batch* batch, const el_batch_params* params);
 /* Autotuner: coordinate descent over bvh_width, queue_cap, block and refill,
- * each trial rendering every scene at min(spp, trial_spp) samples; writes
- * the fastest setting into *params (flags are kept). */
+ * each trial rendering every scene at min(spp, trial_spp) samples; a value
+ * replaces the current one only if its trial is at least 2 % faster.  Writes
+ * the chosen setting into *params (flags are kept). */
 int  el_batch_tune(el_batch* batch, el_batch_params* params, uint32_t trial_spp);
 /* Mean RGBA [H][W][4] and accepted-sample counts of scene `scene`. */
 int  el_batch_read(el_batch* batch, int32_t scene, float* rgba, uint32_t* samples);

@@ -1259,6 +1259,9 @@ extern "C" int el_batch_render(el_batch* b, const el_batch_params* p) {
 // Coordinate descent over BVH width, queue capacity, traversal block size
 // and refill threshold, each trial rendering every scene at min(spp,
 // trial_spp) samples; keeps the fastest setting of each knob in turn.
+// A setting replaces the current best only if its trial is this much faster.
+constexpr double kTuneMargin = 0.98;
+
 extern "C" int el_batch_tune(el_batch* b, el_batch_params* io, uint32_t trial_spp) {
     if (!b || !io) return bfail(b, EL_ERR_ARG, "null argument");
     if (trial_spp == 0) trial_spp = 2;
@@ -1291,7 +1294,7 @@ extern "C" int el_batch_tune(el_batch* b, el_batch_params* io, uint32_t trial_sp
             double ms = 0;
             int r = trial(p, &ms);
             if (r) return r;
-            if (ms < bestMs) {
+            if (ms < bestMs * kTuneMargin) {   // trials closer than the margin are noise
                 bestMs = ms;
                 best = p;
             }

@@ -71,6 +71,27 @@ __device__ __forceinline__ void blockFlush(unsigned long long* counters, const i
     }
 }
 
+// Chunked work of a traversal launch over n queue items (counter set `kind`):
+// chunks of kChunkRays, smaller (down to 64) when the queue would not give
+// every wave of the grid eight of them.
+__device__ __forceinline__ WorkSource chunkSource(uint32_t* qcounters, int kind, uint32_t n) {
+    WorkSource ws;
+    ws.begin = ws.end = 0;
+    ws.ctr = qcounters + kQChunk + kind * kXcds * kChunkStride;
+    ws.n = n;
+    const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t fit = (uint32_t)(((uint64_t)n / ((uint64_t)waves * 8)) & ~63ull);
+    ws.chunk = fit < 64 ? 64u : (fit > kChunkRays ? kChunkRays : fit);
+    ws.stride = kChunkStride;
+    ws.parts = kXcds;
+    ws.home = xcdOf();
+    return ws;
+}
+// Zeroes the chunk counters of `kind` (threads 0..kXcds-1 of block 0 call).
+__device__ __forceinline__ void resetChunks(uint32_t* qcounters, int kind) {
+    if (blockIdx.x == 0 && threadIdx.x < kXcds) qcounters[kQChunk + (kind * kXcds + threadIdx.x) * kChunkStride] = 0;
+}
+
 // Dense index i over an XCD-segmented queue -> entry index.
 struct QueueMap {
     uint32_t pre[kXcds + 1];
@@ -168,8 +189,9 @@ EL_KERNEL(EL_WPE_TRAV) void k_trace(DevScene s, const float* __restrict__ rays, 
                                     uint32_t gthreads, uint32_t refill, unsigned long long* counters) {
     TravStack st = makeStack(s, gstack, gthreads);
     TraceCounters cnt{0, 0};
-    uint32_t begin, end, traced = 0;
-    waveRange(n, begin, end);
+    uint32_t traced = 0;
+    WorkSource ws{};
+    waveRange(n, ws.begin, ws.end);
     auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
                     BvhPtr&) {
         const float* r = rays + (uint64_t)i * 6;
@@ -194,7 +216,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_trace(DevScene s, const float* __restrict__ rays, 
             }
         }
     };
-    traceWaveRange<W, ANY, COUNT>(s, st, begin, end, refill, load, store, cnt, traced);
+    traceWaveRange<W, ANY, COUNT>(s, st, ws, refill, load, store, cnt, traced);
     const int slots[3] = {kStatNodes, kStatTris, kStatOverflow};
     const uint32_t vals[3] = {cnt.nodes, cnt.tris, (uint32_t)st.overflow};
     blockFlush<3>(counters, slots, vals);
@@ -624,6 +646,7 @@ EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
         Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(a.scene));
     }
     if (blockIdx.x == 0 && threadIdx.x < kXcds) q.counters[kQRay0 + threadIdx.x] = threadIdx.x == 0 ? a.genCount : 0u;
+    resetChunks(q.counters, kChunkExt);
 }
 
 // Scene addressing of the wavefront kernels.  OneScene: the kernel argument
@@ -661,8 +684,10 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
         q.counters[kQShadowH + threadIdx.x] = 0;
         q.counters[kQShadowP + threadIdx.x] = 0;
     }
-    uint32_t begin, end, rays = 0;
-    waveRange(n, begin, end);
+    resetChunks(q.counters, kChunkShH);   // for the shadow launches that follow
+    resetChunks(q.counters, kChunkShP);
+    uint32_t rays = 0;
+    const WorkSource ws = chunkSource(q.counters, kChunkExt, n);
     const uint32_t segSize = q.segSize;
     auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
                     BvhPtr& bvh) {
@@ -684,7 +709,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
     auto store = [&](uint32_t i, const TraceResult& h, bool) {
         q.hits[qm.entry(i, segSize)] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
     };
-    traceWaveRange<W, false, COUNT, P::kMany>(s0, st, begin, end, refill, load, store, cnt, rays);
+    traceWaveRange<W, false, COUNT, P::kMany>(s0, st, ws, refill, load, store, cnt, rays);
     const int slots[4] = {kStatRays, kStatOverflow, kStatNodes, kStatTris};
     const uint32_t vals[4] = {rays, (uint32_t)st.overflow, cnt.nodes, cnt.tris};
     blockFlush<4>(counters, slots, vals);
@@ -712,6 +737,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
     uint32_t* const outCnt[3] = {countOut, q.counters + kQShadowH, q.counters + kQShadowP};
     const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = __lane_id();
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    resetChunks(q.counters, kChunkExt);   // for the next bounce's extend launch
     uint32_t pending = 0;   // hits in the pool (block-uniform)
     for (uint32_t b = blockIdx.x * blockDim.x;; b += stride) {
         const bool more = b < n;
@@ -830,8 +856,8 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
     QueueMap qm;
     qm.load(countIn);
     const uint32_t n = qm.total();
-    uint32_t begin, end, traced = 0;
-    waveRange(n, begin, end);
+    uint32_t traced = 0;
+    const WorkSource ws = chunkSource(q.counters, sh == q.shadowH ? kChunkShH : kChunkShP, n);
     const uint32_t segSize = q.segSize;
     auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
                     BvhPtr& bvh) {
@@ -858,7 +884,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
         const float4 L = q.pathL[slot];
         q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
     };
-    traceWaveRange<W, true, false, P::kMany>(s0, st, begin, end, refill, load, store, cnt, traced);
+    traceWaveRange<W, true, false, P::kMany>(s0, st, ws, refill, load, store, cnt, traced);
     const int slots[1] = {kStatOverflow};
     const uint32_t vals[1] = {(uint32_t)st.overflow};
     blockFlush<1>(counters, slots, vals);
@@ -943,6 +969,7 @@ EL_KERNEL(1) void k_bt_camera(ManyScenes sc, const BatchSeg* __restrict__ segs, 
         Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(g.scene));
     }
     if (blockIdx.x == 0 && threadIdx.x < kXcds) q.counters[kQRay0 + threadIdx.x] = threadIdx.x == 0 ? genCount : 0u;
+    resetChunks(q.counters, kChunkExt);
 }
 
 // Accumulation of a batch generation: one thread per distinct (scene, pixel)

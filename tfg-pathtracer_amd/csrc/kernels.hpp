@@ -30,7 +30,12 @@ constexpr int kStatSlots = 8;
 constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5;
 
 // Queue counters: kXcds per queue.
-constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadowH = 2 * kXcds, kQShadowP = 3 * kXcds, kQCounters = 4 * kXcds;
+constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadowH = 2 * kXcds, kQShadowP = 3 * kXcds;
+// Work-chunk counters of the traversal launches (extend, HDRI shadow, light
+// shadow), one per XCD share, each on its own 128-byte line.
+constexpr int kChunkStride = 32, kChunkExt = 0, kChunkShH = 1, kChunkShP = 2;
+constexpr int kQChunk = 4 * kXcds, kQCounters = kQChunk + 3 * kXcds * kChunkStride;
+constexpr uint32_t kChunkRays = 256;   // rays per work chunk (a multiple of 64)
 
 struct RenderArgs {
     uint32_t k0, k1;        // RNG key (seed)

@@ -893,24 +893,64 @@ struct BvhPtr {
     const float4* tris;
 };
 
+// The items a wave traces: a fixed range [begin, end), or (ctr set) chunks
+// of `chunk` items handed out at run time.  Chunk c belongs to share
+// c % parts; share x is counted by ctr[x * stride].  A wave draws from its
+// home share first, then from the others, so waves that meet cheap rays take
+// more chunks and a launch ends when the work does, not when its slowest
+// fixed range does.  Every counter starts the launch at 0.  A share is
+// skipped without an atomic once a coherent read shows it drawn out (a read
+// never runs ahead of the counter, so no chunk is lost).
+struct WorkSource {
+    uint32_t begin, end;
+    uint32_t* ctr;
+    uint32_t n, chunk, stride, parts, home;
+};
+
 template <int W, bool ANY, bool COUNT, bool PER_RAY_BVH = false, class Load, class Store>
-__device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st, uint32_t begin, uint32_t end,
+__device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st, const WorkSource& ws,
                                                uint32_t refill, Load&& load, Store&& store, TraceCounters& cnt,
                                                uint32_t& traced) {
     BvhPtr bvh{(const uint4*)s.nodes, s.tris};
     const uint32_t lane = __lane_id();
     const unsigned long long ltMask = lane ? (~0ull >> (64 - lane)) : 0ull;
-    uint32_t cursor = begin;   // wave-uniform
+    uint32_t cursor = ws.begin, end = ws.end;   // wave-uniform
+    uint32_t tried = 0;                         // dynamic: shares found empty
+    bool more = true;                           // wave-uniform: items may remain
     bool live = false;
     uint32_t item = 0;
     TRay ray{};
     TraceResult r{};
     int cur = -1, leaf = -1;
     while (true) {
-        // ---- refill idle lanes from the wave's range
+        // ---- refill idle lanes from the wave's range or the next chunk
         const unsigned long long idle = __ballot(!live);
         const uint32_t nIdle = (uint32_t)__popcll(idle);
-        if (cursor < end && (nIdle >= refill || nIdle == 64)) {
+        const bool want = nIdle >= refill || nIdle == 64;
+        if (want && cursor >= end) {
+            more = false;
+            while (ws.ctr && tried < ws.parts) {
+                const uint32_t x = (ws.home + tried) % ws.parts;
+                uint32_t* const c = ws.ctr + x * ws.stride;
+                const uint32_t seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (((uint64_t)x + (uint64_t)ws.parts * seen) * ws.chunk >= ws.n) {
+                    tried++;
+                    continue;
+                }
+                uint32_t k = 0;
+                if (lane == (uint32_t)(__ffsll((long long)__ballot(true)) - 1)) k = atomicAdd(c, 1u);
+                k = __builtin_amdgcn_readfirstlane(k);
+                const uint64_t first = ((uint64_t)x + (uint64_t)ws.parts * k) * ws.chunk;
+                if (first < ws.n) {
+                    cursor = (uint32_t)first;
+                    end = (uint32_t)(first + ws.chunk < ws.n ? first + ws.chunk : ws.n);
+                    more = true;
+                    break;
+                }
+                tried++;
+            }
+        }
+        if (want && more) {
             if (!live) {
                 uint32_t i = cursor + (uint32_t)__popcll(idle & ltMask);
                 if (i < end) {
@@ -937,8 +977,8 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                 }
             }
             cursor += nIdle;
-        } else if (nIdle == 64) {
-            break;   // range exhausted and every lane finished
+        } else if (nIdle == 64 && !more) {
+            break;   // no items left and every lane finished
         }
         // ---- inner nodes: descend until every live lane holds a leaf
         while (true) {
