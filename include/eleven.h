@@ -149,6 +149,8 @@ typedef struct {
     double   adjoint_ms;       /* device time of the last el_adjoint         */
     uint64_t adjoint_rays;     /* BVH queries of the last el_adjoint (primal
                                   walk + replay)                             */
+    uint64_t texels_8bit;      /* texels read from the 8-bit copies (textures
+                                  from 8-bit files, not since set_texture)   */
 } el_stats;
 
 /* Adjoint of the accumulated image (no counterpart in the reference, which

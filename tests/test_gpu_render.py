@@ -194,3 +194,28 @@ def test_zero_throughput_culling_is_exact(gpu_ctx, scenes, kind):
         assert np.array_equal(ca, cb)
         assert np.array_equal(a, b)
         assert ra <= rb
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["spheres_sky_tex", "terrain_bokeh"])
+def test_8bit_texture_copy_is_exact(gpu_ctx, scenes, kind):
+    """Textures loaded from 8-bit files are read from a 4-byte-per-texel copy
+    through the loader's byte -> value tables.  Writing the same values back
+    with set_texture moves the texture to its FP32 copy: the images of the
+    two storages must be identical bit for bit."""
+    print(f"[render] 8-bit texture copy vs FP32: {kind}", flush=True)
+    scene = scenes[kind]
+    assert scene.view.texture_count > 0
+    texels = sum(scene.view.textures[i].width * scene.view.textures[i].height
+                 for i in range(scene.view.texture_count))
+    for mode in (eleven.MODE_WAVEFRONT, eleven.MODE_PLAIN):
+        a, ca, ra, _, _ = _render(gpu_ctx, scene, 4, 17, mode)
+        assert gpu_ctx.stats().texels_8bit == texels   # every texture came from an 8-bit file
+        for i in range(scene.view.texture_count):
+            gpu_ctx.set_texture(i, np.ascontiguousarray(scene.texture(i), np.float32))
+        assert gpu_ctx.stats().texels_8bit == 0
+        gpu_ctx.clear()
+        gpu_ctx.render(4, depth=5, seed=17, mode=mode)
+        b, cb, rb, _ = gpu_ctx.read()
+        assert np.array_equal(ca, cb) and ra == rb
+        assert np.array_equal(a, b[..., :3].astype(np.float64))

@@ -23,6 +23,7 @@
 #include "hdri_table.hpp"
 #include "kernels.hpp"
 #include "pt_device.hpp"
+#include "scene.hpp"
 
 using namespace eleven;
 using eleven::dev::DevScene;
@@ -87,8 +88,9 @@ struct el_ctx {
     el_camera cam{};
     std::vector<dev::DevMaterial> mats;
     std::vector<dev::DevTex> texs;
+    std::vector<uint64_t> texF32;   // FP32-pool offset of each texture (el_set_texture)
 
-    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad, dEnvTab;
+    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad, dEnvTab, dTexels8, dTexLut;
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
     DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters;
     size_t wfCap = 0, wfQcap = 0;
@@ -185,6 +187,24 @@ const char* el_last_error(const el_ctx* c) { return c ? c->err.c_str() : "null c
 }  // extern "C"
 
 // Pack and upload the wide BVH of the current width.
+// 8-bit copy of an RGB FP32 texture: R | G << 8 | B << 16 per texel, each
+// byte the index of the value in `table` (bit-exact); false if some value is
+// not in the table.  The table is non-decreasing (the loader's fastPow of
+// i / 255), so each value is found by binary search.
+static bool packLdr(const float* rgb, size_t n, const float (&table)[256], uint32_t* out) {
+    for (size_t i = 0; i < n; i++) {
+        uint32_t w = 0;
+        for (int ch = 0; ch < 3; ch++) {
+            const float v = rgb[3 * i + ch];
+            const float* p = std::lower_bound(table, table + 256, v);
+            if (p == table + 256 || std::memcmp(p, &v, 4) != 0) return false;
+            w |= (uint32_t)(p - table) << (8 * ch);
+        }
+        out[i] = w;
+    }
+    return true;
+}
+
 // Material values as the kernels read them (texture ids are set by the
 // caller); for untextured roughness and metallic also generateHitData's
 // powf(x, 2.2f) (kernel.cu:113-114), once per material.
@@ -291,10 +311,17 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     }
     EL_HIP(c, upload(c->dShade, shade.data(), shade.size() * sizeof(float), c->stream));
 
-    // textures: one pool, the environment map last
+    // textures: one FP32 pool, the environment map last.  A texture whose
+    // every value is an entry of one of the loader's byte -> value tables
+    // (an 8-bit file) also gets an 8-bit copy, which the kernels read.
     std::vector<float> pool;
+    std::vector<uint32_t> pool8;
+    float tables[2][256];
+    ldrTable(sRGB, tables[0]);
+    ldrTable(LINEAR, tables[1]);
     c->texs.clear();
-    auto addTex = [&](const el_texture& t) {
+    c->texF32.clear();
+    auto addTex = [&](const el_texture& t, bool mayPack) {
         dev::DevTex d{};
         d.w = t.width;
         d.h = t.height;
@@ -303,13 +330,29 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
         d.xOffset = t.x_offset;
         d.yOffset = t.y_offset;
         d.filter = t.filter;
+        d.format = dev::kTexF32;
         d.offset = pool.size();
-        pool.insert(pool.end(), t.data, t.data + (size_t)t.width * t.height * 3);
+        const size_t n = (size_t)t.width * t.height;
+        pool.insert(pool.end(), t.data, t.data + n * 3);
+        for (int k = 0; mayPack && k < 2; k++) {
+            std::vector<uint32_t> words(n);
+            if (!packLdr(t.data, n, tables[k], words.data())) continue;
+            d.format = dev::kTexLdr + k;
+            d.offset = pool8.size();
+            pool8.insert(pool8.end(), words.begin(), words.end());
+            break;
+        }
         return d;
     };
-    for (uint32_t i = 0; i < v->texture_count; i++) c->texs.push_back(addTex(v->textures[i]));
-    dev::DevTex env = addTex(v->hdri);
+    for (uint32_t i = 0; i < v->texture_count; i++) {
+        c->texF32.push_back(pool.size());
+        c->texs.push_back(addTex(v->textures[i], true));
+    }
+    dev::DevTex env = addTex(v->hdri, false);
     EL_HIP(c, upload(c->dTexels, pool.data(), pool.size() * sizeof(float), c->stream));
+    EL_HIP(c, upload(c->dTexels8, pool8.data(), pool8.size() * sizeof(uint32_t), c->stream));
+    EL_HIP(c, upload(c->dTexLut, tables, sizeof tables, c->stream));
+    c->stats.texels_8bit = pool8.size();
     EL_HIP(c, upload(c->dTexs, c->texs.data(), c->texs.size() * sizeof(dev::DevTex), c->stream));
 
     // materials
@@ -363,6 +406,8 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     s.mats = c->dMats.as<dev::DevMaterial>();
     s.texs = c->dTexs.as<dev::DevTex>();
     s.texels = c->dTexels.as<float>();
+    s.texels8 = c->dTexels8.as<uint32_t>();
+    s.texLut = c->dTexLut.as<float>();
     s.lights = c->dLights.as<float>();
     s.envCdf = c->dEnvCdf.as<float>();
     s.envGuide = c->dEnvGuide.as<int32_t>();
@@ -748,10 +793,17 @@ int el_set_texture(el_ctx* c, int32_t id, const float* rgb) {
     if (!c || !rgb) return EL_ERR_ARG;
     if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
     if (id < 0 || (size_t)id >= c->texs.size()) return fail(c, EL_ERR_ARG, "texture id out of range");
-    const dev::DevTex& t = c->texs[id];
+    // new values go to the FP32 copy, which the texture reads from then on
+    dev::DevTex& t = c->texs[id];
     EL_HIP(c, hipSetDevice(c->device));
-    EL_HIP(c, hipMemcpyAsync(c->dTexels.as<float>() + t.offset, rgb, (size_t)t.w * t.h * 3 * sizeof(float),
+    EL_HIP(c, hipMemcpyAsync(c->dTexels.as<float>() + c->texF32[id], rgb, (size_t)t.w * t.h * 3 * sizeof(float),
                              hipMemcpyHostToDevice, c->stream));
+    if (t.format != dev::kTexF32) {
+        c->stats.texels_8bit -= (uint64_t)t.w * t.h;
+        t.format = dev::kTexF32;
+        t.offset = c->texF32[id];
+        EL_HIP(c, hipMemcpyAsync(c->dTexs.as<dev::DevTex>() + id, &t, sizeof t, hipMemcpyHostToDevice, c->stream));
+    }
     EL_HIP(c, hipStreamSynchronize(c->stream));
     return EL_OK;
 }

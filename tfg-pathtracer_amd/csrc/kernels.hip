@@ -36,7 +36,7 @@ constexpr int kMaxWaves = kMaxBlock / 64;
 #define EL_WPE_TRAV 8
 #endif
 #ifndef EL_WPE_SHADE
-#define EL_WPE_SHADE 1
+#define EL_WPE_SHADE 3
 #endif
 #ifndef EL_WPE_PLAIN
 #define EL_WPE_PLAIN 4
@@ -727,6 +727,9 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                            unsigned long long* counters) {
     __shared__ uint32_t s_pool[2 * kMaxBlock];
     __shared__ uint32_t s_wcnt[kMaxWaves];
+    __shared__ float s_lut[512];   // the byte -> value tables (the same for every scene)
+    for (uint32_t i = threadIdx.x; i < 512; i += blockDim.x) s_lut[i] = sc.any().texLut[i];
+    __syncthreads();
     QueueMap qm;
     qm.load(countIn);
     const uint32_t n = qm.total();
@@ -792,7 +795,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
             float4 Tf = q.pathT[slot];
             SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
-            HitData hd = hitData(s, sh);
+            HitData hd = hitData(s, sh, s_lut);
             uint32_t k0 = a.k0, k1 = a.k1;
             if (P::kMany) {
                 k0 = s.seedLo;

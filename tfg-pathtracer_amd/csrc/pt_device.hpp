@@ -104,12 +104,17 @@ struct PathRng {
 };
 
 // ----------------------------------------------------------- scene layout --
+// Texel storage: kTexF32 = RGB FP32 in the float pool (offset = first
+// float); kTexLdr + k = one 32-bit word per texel in the 8-bit pool (offset =
+// first texel), bytes R, G, B indexing the loader's byte -> value table k
+// (0 sRGB, 1 linear): the same FP32 values in a third of the bytes.
+constexpr int32_t kTexF32 = 0, kTexLdr = 1;
 struct DevTex {
     int32_t w, h;
     float xTile, yTile, xOffset, yOffset;
     int32_t filter;
-    int32_t pad;
-    uint64_t offset;   // first float of this texture in the texel pool
+    int32_t format;
+    uint64_t offset;
 };
 
 struct DevMaterial {
@@ -141,6 +146,8 @@ struct DevScene {
     const DevMaterial* mats;
     const DevTex* texs;
     const float* texels;           // pool of all textures (RGB FP32)
+    const uint32_t* texels8;       // 8-bit copies of the LDR textures (DevTex::format)
+    const float* texLut;           // 2 x 256 byte -> value tables (sRGB, linear)
     const float* lights;           // 6 floats per light: position, radiance
     const float* envCdf;           // HDRI CDF (L + 1)
     const int32_t* envGuide;       // guide table (2^envLog2G + 1 pairs)
@@ -162,16 +169,27 @@ struct DevScene {
 };
 
 // --------------------------------------------------------------- textures --
+struct TexPools {
+    const float* f32;
+    const uint32_t* u8;
+    const float* lut;   // the byte -> value tables (global or an LDS copy)
+};
+__device__ __forceinline__ TexPools texPools(const DevScene& s, const float* lut = nullptr) {
+    return TexPools{s.texels, s.texels8, lut ? lut : s.texLut};
+}
 // Texture::getValueFromCoordinates (Texture.hpp:95-106).
-__device__ __forceinline__ v3 texel(const float* __restrict__ pool, const DevTex& t, int x, int y) {
+__device__ __forceinline__ v3 texel(const TexPools& P, const DevTex& t, int x, int y) {
     x = (int)(t.xTile * (x + t.xOffset * t.w)) % t.w;
     y = (int)(t.yTile * (y + t.yOffset * t.h)) % t.h;
     // Out-of-range coordinates (negative uv) are undefined in the reference;
     // wrap them into the texture instead of reading out of bounds.
     if (x < 0) x += t.w;
     if (y < 0) y += t.h;
-    const float* p = pool + t.offset + 3 * ((uint64_t)y * t.w + x);
-    return ld3(p);
+    const uint64_t i = (uint64_t)y * t.w + x;
+    if (t.format == kTexF32) return ld3(P.f32 + t.offset + 3 * i);
+    const uint32_t v = P.u8[t.offset + i];
+    const float* L = P.lut + 256 * (t.format - kTexLdr);
+    return mk(L[v & 255u], L[(v >> 8) & 255u], L[(v >> 16) & 255u]);
 }
 // Index (texel units, within the texture) that texel() reads for (x, y).
 __device__ __forceinline__ uint32_t texelIndex(const DevTex& t, int x, int y) {
@@ -181,11 +199,11 @@ __device__ __forceinline__ uint32_t texelIndex(const DevTex& t, int x, int y) {
     if (y < 0) y += t.h;
     return (uint32_t)y * (uint32_t)t.w + (uint32_t)x;
 }
-__device__ __forceinline__ v3 texUV(const float* pool, const DevTex& t, float u, float v) {
+__device__ __forceinline__ v3 texUV(const TexPools& pool, const DevTex& t, float u, float v) {
     return texel(pool, t, (int)(u * t.w), (int)(v * t.h));
 }
 // Texture.hpp:114-135
-__device__ __forceinline__ v3 texBilinear(const float* pool, const DevTex& t, float u, float v) {
+__device__ __forceinline__ v3 texBilinear(const TexPools& pool, const DevTex& t, float u, float v) {
     float x = u * t.w, y = v * t.h;
     float t1x = floorf(x), t1y = floorf(y);
     float t2x = t1x + 1, t2y = t1y + 1;
@@ -194,7 +212,7 @@ __device__ __forceinline__ v3 texBilinear(const float* pool, const DevTex& t, fl
     v3 v3_ = texel(pool, t, (int)t1x, (int)t2y), v4 = texel(pool, t, (int)t2x, (int)t2y);
     return lerp3(lerp3(v1, v2, a), lerp3(v3_, v4, a), b);
 }
-__device__ __forceinline__ v3 texFiltered(const float* pool, const DevTex& t, float u, float v) {
+__device__ __forceinline__ v3 texFiltered(const TexPools& pool, const DevTex& t, float u, float v) {
     return t.filter == 1 ? texBilinear(pool, t, u, v) : texUV(pool, t, u, v);
 }
 
@@ -530,21 +548,23 @@ __device__ __forceinline__ SurfaceHit surfaceAt(const DevScene& s, int tri, floa
 }
 
 // generateHitData (kernel.cu:54-119)
-__device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& h) {
+// `lut`: an LDS copy of s.texLut, or null for the global one.
+__device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& h, const float* lut = nullptr) {
     const DevMaterial& m = s.mats[h.material];
+    const TexPools P = texPools(s, lut);
     HitData d;
-    d.albedo = m.albedoTex < 0 ? ld3(m.albedo) : texFiltered(s.texels, s.texs[m.albedoTex], h.tu, h.tv);
-    d.emission = m.emissionTex < 0 ? ld3(m.emission) : texFiltered(s.texels, s.texs[m.emissionTex], h.tu, h.tv);
+    d.albedo = m.albedoTex < 0 ? ld3(m.albedo) : texFiltered(P, s.texs[m.albedoTex], h.tu, h.tv);
+    d.emission = m.emissionTex < 0 ? ld3(m.emission) : texFiltered(P, s.texs[m.emissionTex], h.tu, h.tv);
     // powf(x, 2.2f) of kernel.cu:113-114: for untextured parameters computed
     // once per material at upload
     d.roughness = m.roughnessTex < 0 ? m.roughnessEff
-                                     : powf(texFiltered(s.texels, s.texs[m.roughnessTex], h.tu, h.tv).x, 2.2f);
+                                     : powf(texFiltered(P, s.texs[m.roughnessTex], h.tu, h.tv).x, 2.2f);
     d.metallic = m.metallicTex < 0 ? m.metallicEff
-                                   : powf(texFiltered(s.texels, s.texs[m.metallicTex], h.tu, h.tv).x, 2.2f);
+                                   : powf(texFiltered(P, s.texs[m.metallicTex], h.tu, h.tv).x, 2.2f);
     if (m.normalTex < 0) {
         d.normal = h.normal;
     } else {
-        v3 c = texUV(s.texels, s.texs[m.normalTex], h.tu, h.tv);
+        v3 c = texUV(P, s.texs[m.normalTex], h.tu, h.tv);
         v3 ln = c * 2.0f - mk(1, 1, 1);
         d.normal = normalized(ln.x * h.tangent - ln.y * h.bitangent + ln.z * h.normal);
     }
@@ -567,7 +587,7 @@ __device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& 
 __device__ __forceinline__ v3 envLookup(const DevScene& s, v3 d) {
     float u, v;
     sphericalMapping(d * -1.0f, u, v);
-    return texFiltered(s.texels, s.env, u, v);
+    return texFiltered(texPools(s), s.env, u, v);
 }
 
 // HDRI NEE sample (kernel.cu:236-254 with HDRI.hpp:145-162, Texture.hpp:177-207).
@@ -595,10 +615,10 @@ __device__ __forceinline__ EnvSample envSampleAt(const DevScene& s, int k) {
     float a = sqrtf(1 - py * py);
     EnvSample e;
     e.dir = normalized(mk(a * px, py, a * pz)) * -1.0f;
-    e.value = texUV(s.texels, t, iu, iv);
+    e.value = texUV(texPools(s), t, iu, iv);
     // HDRI::pdf((int)(iu*W), (int)(iv*H))
     int qx = (int)(iu * t.w), qy = (int)(iv * t.h);
-    v3 dv = texel(s.texels, t, qx, qy);
+    v3 dv = texel(texPools(s), t, qx, qy);
     float th = (((float)qy / (float)t.h)) * EL_PI;
     e.pdf = ((dv.x + dv.y + dv.z) / s.envRadianceSum) * (float)(t.w * t.h) / (2.0f * EL_PI * sinf(th));
     e.shadowDir = normalized(e.dir);

@@ -140,6 +140,9 @@ struct LoadError {
 
 Scene loadScene(const std::string& dir);
 Texture loadTexture(const std::string& path, CS colorSpace);
+// The value an 8-bit texel byte i loads as, per colour space (stb_image.h
+// ldr-to-hdr as modified by the reference: fastPow(i / 255.0f, gamma)).
+void ldrTable(CS colorSpace, float out[256]);
 HDRI loadHDRIFile(const std::string& path);
 HDRI makeConstantHDRI(Vector3 color);
 // HDRI::generateCDF (HDRI.hpp:107-128): sequential FP32 sums, row-major.

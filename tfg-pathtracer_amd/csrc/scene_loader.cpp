@@ -430,12 +430,16 @@ Texture loadTexture(const std::string& path, CS colorSpace) {
     t.width = w;
     t.height = h;
     t.data.resize(rgb.size());
-    // stb_image.h:1855-1870 (modified): fastPow(v/255.0f, gamma) * scale.
-    const double gamma = colorSpace == sRGB ? (double)2.2f : 1.0;
     float lut[256];
-    for (int i = 0; i < 256; i++) lut[i] = (float)(fastPow((double)(i / 255.0f), gamma) * 1.0f);
+    ldrTable(colorSpace, lut);
     for (size_t i = 0; i < rgb.size(); i++) t.data[i] = lut[rgb[i]];
     return t;
+}
+
+// stb_image.h:1855-1870 (modified): fastPow(v/255.0f, gamma) * scale.
+void ldrTable(CS colorSpace, float out[256]) {
+    const double gamma = colorSpace == sRGB ? (double)2.2f : 1.0;
+    for (int i = 0; i < 256; i++) out[i] = (float)(fastPow((double)(i / 255.0f), gamma) * 1.0f);
 }
 
 Vector3 texelAt(const Texture& t, int x, int y) {
