@@ -208,6 +208,12 @@ int  el_get_stats(el_ctx* ctx, el_stats* out);
 int  el_set_materials(el_ctx* ctx, const el_material* materials, uint32_t count);
 /* Replace the texels of texture `tex_id` (same size as uploaded). */
 int  el_set_texture(el_ctx* ctx, int32_t tex_id, const float* rgb);
+/* Replace the camera (the el_camera fields of Camera.hpp:10-23; the
+ * resolution may change) and clear the accumulation, so one uploaded scene
+ * is rendered from several views (multi-view inverse rendering).  The
+ * reference fixes its camera when the scene is loaded (SceneLoader.hpp:24-45);
+ * this entry point has no counterpart there. */
+int  el_set_camera(el_ctx* ctx, const el_camera* camera);
 
 /* Path-replay adjoint (Vicini et al. 2021, detached sampling) of the image
  * currently accumulated in the context (every el_render since el_clear,

@@ -205,6 +205,33 @@ static bool packLdr(const float* rgb, size_t n, const float (&table)[256], uint3
     return true;
 }
 
+// The device camera: the host fields plus the six trig values the reference
+// evaluates per ray (kernel.cu:304-306) after rotation *= (PI / 180.0) as
+// Vector3 *= float (kernel.cu:301).
+static dev::DevCamera devCamera(const el_camera& cam) {
+    dev::DevCamera dc{};
+    dc.xRes = cam.x_res;
+    dc.yRes = cam.y_res;
+    dc.focalLength = cam.focal_length;
+    dc.sensorWidth = cam.sensor_width;
+    dc.sensorHeight = cam.sensor_height;
+    dc.aperture = cam.aperture;
+    dc.focusDistance = cam.focus_distance;
+    const float toRad = (float)(3.14159265358979323846f / 180.0);
+    const float rx = cam.rotation.x * toRad, ry = cam.rotation.y * toRad, rz = cam.rotation.z * toRad;
+    dc.cosRot[0] = std::cos(rx);
+    dc.sinRot[0] = std::sin(rx);
+    dc.cosRot[1] = std::cos(ry);
+    dc.sinRot[1] = std::sin(ry);
+    dc.cosRot[2] = std::cos(rz);
+    dc.sinRot[2] = std::sin(rz);
+    dc.pos[0] = cam.position.x;
+    dc.pos[1] = cam.position.y;
+    dc.pos[2] = cam.position.z;
+    dc.bokeh = cam.bokeh;
+    return dc;
+}
+
 // Material values as the kernels read them (texture ids are set by the
 // caller); for untextured roughness and metallic also generateHitData's
 // powf(x, 2.2f) (kernel.cu:113-114), once per material.
@@ -420,34 +447,27 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     s.envLog2G = hs.log2G;
     s.nLights = (int32_t)v->light_count;
     s.nTris = (int32_t)v->tri_count;
-    dev::DevCamera& dc = s.cam;
-    dc.xRes = v->camera.x_res;
-    dc.yRes = v->camera.y_res;
-    dc.focalLength = v->camera.focal_length;
-    dc.sensorWidth = v->camera.sensor_width;
-    dc.sensorHeight = v->camera.sensor_height;
-    dc.aperture = v->camera.aperture;
-    dc.focusDistance = v->camera.focus_distance;
-    // rotation *= (PI / 180.0) as Vector3 *= float (kernel.cu:301), then the
-    // six trig values the reference evaluates per ray (kernel.cu:304-306)
-    const float toRad = (float)(3.14159265358979323846f / 180.0);
-    const float rx = v->camera.rotation.x * toRad, ry = v->camera.rotation.y * toRad,
-                rz = v->camera.rotation.z * toRad;
-    dc.cosRot[0] = std::cos(rx);
-    dc.sinRot[0] = std::sin(rx);
-    dc.cosRot[1] = std::cos(ry);
-    dc.sinRot[1] = std::sin(ry);
-    dc.cosRot[2] = std::cos(rz);
-    dc.sinRot[2] = std::sin(rz);
-    dc.pos[0] = v->camera.position.x;
-    dc.pos[1] = v->camera.position.y;
-    dc.pos[2] = v->camera.position.z;
-    dc.bokeh = v->camera.bokeh;
+    s.cam = devCamera(v->camera);
     EL_HIP(c, launchEnvTable(c->stream, s, c->dEnvTab.as<float4>()));
     rc = clearAccum(c);
     if (rc) return rc;
     c->hasScene = true;
     return EL_OK;
+}
+
+int el_set_camera(el_ctx* c, const el_camera* cam) {
+    if (!c || !cam) return EL_ERR_ARG;
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (cam->x_res == 0 || cam->y_res == 0) return fail(c, EL_ERR_ARG, "camera resolution is zero");
+    if ((uint64_t)cam->x_res * cam->y_res > (1ull << 28)) return fail(c, EL_ERR_RANGE, "resolution above 2^28 pixels");
+    EL_HIP(c, hipSetDevice(c->device));
+    const size_t npx = (size_t)cam->x_res * cam->y_res;
+    EL_HIP(c, c->dAccum.ensure(npx * sizeof(float4)));
+    EL_HIP(c, c->dCounts.ensure(npx * sizeof(uint32_t)));
+    if (cam->x_res != c->cam.x_res || cam->y_res != c->cam.y_res) c->dAov.release();
+    c->cam = *cam;
+    c->ds.cam = devCamera(*cam);
+    return clearAccum(c);
 }
 
 int el_clear(el_ctx* c) {

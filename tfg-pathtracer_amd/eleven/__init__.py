@@ -120,6 +120,7 @@ _SIGS = {
     "el_get_stats": (C.c_int, [_P, C.POINTER(el_stats)]),
     "el_set_materials": (C.c_int, [_P, C.POINTER(el_material), C.c_uint32]),
     "el_set_texture": (C.c_int, [_P, C.c_int32, _P]),
+    "el_set_camera": (C.c_int, [_P, _P]),
     "el_adjoint": (C.c_int, [_P, C.POINTER(el_adjoint_params), _P, _P, _P, C.c_int]),
     "el_batch_create": (C.c_int, [C.POINTER(_P), C.c_int]),
     "el_batch_destroy": (None, [_P]),
@@ -331,6 +332,7 @@ class Context:
         if rc != EL_OK:
             raise ElevenError(rc, "el_create failed (no HIP device?)")
         self.scene = None
+        self.camera = None   # the camera rendered from (the scene's, or set_camera's)
 
     def _check(self, rc):
         if rc != EL_OK:
@@ -350,6 +352,7 @@ class Context:
     def upload(self, scene: SceneData):
         self._check(lib().el_upload_scene(self._h, C.byref(scene.view)))
         self.scene = scene
+        self.camera = scene.camera
 
     def render(self, spp, depth=5, seed=0, mode=MODE_WAVEFRONT, block=0, queue_cap=0, bvh_width=0, flags=0,
                refill=0, rng_scene=0):
@@ -361,7 +364,7 @@ class Context:
         self._check(lib().el_clear(self._h))
 
     def read(self):
-        cam = self.scene.camera
+        cam = self.camera
         rgba = np.zeros((cam.y_res, cam.x_res, 4), np.float32)
         samples = np.zeros((cam.y_res, cam.x_res), np.uint32)
         rays, verts = C.c_uint64(), C.c_uint64()
@@ -370,7 +373,7 @@ class Context:
         return rgba, samples, rays.value, verts.value
 
     def read_aov(self, pass_id):
-        cam = self.scene.camera
+        cam = self.camera
         rgba = np.zeros((cam.y_res, cam.x_res, 4), np.float32)
         self._check(lib().el_read_aov(self._h, pass_id, rgba.ctypes.data))
         return rgba
@@ -390,7 +393,7 @@ class Context:
         every material's (albedo r, g, b, roughness, metallic, specular) and the
         texels of texture `grad_texture`.  Returns (dmat [M][6], dtex [h][w][3]
         or None)."""
-        cam = self.scene.camera
+        cam = self.camera
         di = np.ascontiguousarray(dimage, np.float32).reshape(cam.y_res, cam.x_res, 3)
         m = len(self.scene.materials())
         dmat = np.zeros((m, 6), np.float32)
@@ -406,6 +409,11 @@ class Context:
     def set_texture(self, tex_id, rgb):
         a = np.ascontiguousarray(rgb, np.float32)
         self._check(lib().el_set_texture(self._h, tex_id, a.ctypes.data))
+
+    def set_camera(self, camera: "el_camera"):
+        """Replace the camera (resolution may change) and clear the accumulation."""
+        self._check(lib().el_set_camera(self._h, C.byref(camera)))
+        self.camera = camera
 
     def trace_closest(self, rays):
         r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)

@@ -1,0 +1,159 @@
+"""Multi-view inverse rendering: material parameters and one albedo texture
+recovered from images of several views by Adam on an L2 image loss.
+
+One step renders every view (el_set_camera, el_render), forms dL/dI =
+2 (I - target) from the mean image, runs the path-replay adjoint
+(el_adjoint) and sums the gradients over the views: one complete gradient
+vector (all [M][6] material parameters and the texels of the texture) per
+step.  Each step and view draws its own random numbers (seed + step * views
++ view).
+
+Decorrelation: 2 (I - T) dI/dtheta evaluated on the same paths is biased by
+2 Cov(I, dI/dtheta) -- at a few samples per pixel it drives e.g. albedo
+towards darker values even where E[I] = T.  By default the residual I comes
+from one render and the replayed derivative from a second, independently
+seeded render of the same view, which makes the gradient estimate unbiased.
+With decorrelate=False the gradient is the exact derivative of the measured
+loss of one render (what a finite-difference check with common random
+numbers measures).  The parameters and texels are clamped to their valid
+ranges after every Adam update.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import MODE_WAVEFRONT, Context, SceneData, el_camera, make_camera
+from .autograd import material_params, with_params
+
+# seed offset of the render whose paths the adjoint replays (decorrelation)
+DECORRELATE_SEED = 0x5EED0001
+
+# valid ranges of (albedo r, g, b, roughness, metallic, specular)
+PARAM_LO = np.array([0.0, 0.0, 0.0, 0.05, 0.0, 0.0], np.float32)
+PARAM_HI = np.array([1.0, 1.0, 1.0, 1.0, 1.0, 1.0], np.float32)
+
+
+def orbit_views(base: el_camera, n: int, x_span: float, pitches=(29.0, 35.0), res=None) -> list:
+    """n cameras copied from `base`, their x positions spread evenly over
+    [-x_span, x_span] and their pitch alternating over `pitches`."""
+    out = []
+    for k in range(n):
+        xr, yr = res if res else (base.x_res, base.y_res)
+        x = -x_span + 2 * x_span * k / max(1, n - 1)
+        out.append(make_camera(xr, yr, position=(x, base.position.y, base.position.z),
+                               rotation=(pitches[k % len(pitches)], base.rotation.y, base.rotation.z),
+                               focal_length=base.focal_length, sensor_width=base.sensor_width,
+                               aperture=base.aperture, focus_distance=base.focus_distance, bokeh=bool(base.bokeh)))
+    return out
+
+
+class MultiView:
+    """Forward renders and summed adjoint gradients of one uploaded scene
+    seen from several cameras."""
+
+    def __init__(self, ctx: Context, scene: SceneData, cameras: list, spp: int, depth: int = 6,
+                 grad_texture: int = -1, mode: int = MODE_WAVEFRONT):
+        self.ctx, self.scene, self.cameras = ctx, scene, list(cameras)
+        self.spp, self.depth, self.grad_texture, self.mode = spp, depth, grad_texture, mode
+        self.base = scene.materials()
+
+    def set_params(self, params: np.ndarray, texture: np.ndarray | None = None):
+        self.ctx.set_materials(with_params(self.base, params))
+        if texture is not None:
+            self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
+
+    def render_view(self, v: int, seed: int, spp: int | None = None) -> np.ndarray:
+        self.ctx.set_camera(self.cameras[v])
+        self.ctx.render(spp or self.spp, depth=self.depth, seed=seed, mode=self.mode)
+        return self.ctx.read()[0][..., :3].copy()
+
+    def images(self, params, texture=None, seed: int = 0, spp: int | None = None) -> list:
+        self.set_params(params, texture)
+        return [self.render_view(v, seed + v, spp) for v in range(len(self.cameras))]
+
+    def loss_and_grad(self, params, texture, targets, seed: int, decorrelate: bool = True):
+        """Summed L2 loss over the views and its gradient (dparams [M][6],
+        dtexture [H][W][3] or None)."""
+        self.set_params(params, texture)
+        loss = 0.0
+        dmat = np.zeros_like(np.asarray(params, np.float32))
+        dtex = None
+        for v in range(len(self.cameras)):
+            img = self.render_view(v, seed + v)
+            if decorrelate:
+                self.render_view(v, seed + v + DECORRELATE_SEED)   # the paths the adjoint replays
+            diff = img.astype(np.float64) - targets[v]
+            loss += float(np.sum(diff * diff))
+            dm, dt = self.ctx.adjoint((2.0 * diff).astype(np.float32),
+                                      grad_texture=self.grad_texture if texture is not None else -1)
+            dmat += dm
+            if texture is not None:
+                dtex = dt if dtex is None else dtex + dt
+        return loss, dmat, dtex
+
+
+@dataclass
+class InverseResult:
+    params: np.ndarray
+    texture: np.ndarray | None
+    losses: list = field(default_factory=list)
+    seconds: float = 0.0
+    steps: int = 0
+
+    @property
+    def grads_per_s(self) -> float:
+        return self.steps / self.seconds if self.seconds > 0 else 0.0
+
+
+def optimize(mv: MultiView, params0, texture0, targets, steps: int = 200, lr: float = 1e-2, seed: int = 1000,
+             log=None) -> InverseResult:
+    """Adam (torch.optim) on the parameters and the texture; one gradient
+    vector (all views) per step."""
+    p = torch.tensor(np.asarray(params0, np.float32).copy(), requires_grad=True)
+    tensors = [p]
+    t = None
+    if texture0 is not None:
+        t = torch.tensor(np.asarray(texture0, np.float32).copy(), requires_grad=True)
+        tensors.append(t)
+    opt = torch.optim.Adam(tensors, lr=lr)
+    lo, hi = torch.from_numpy(PARAM_LO), torch.from_numpy(PARAM_HI)
+    res = InverseResult(params=None, texture=None)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        loss, dmat, dtex = mv.loss_and_grad(p.detach().numpy(), None if t is None else t.detach().numpy(), targets,
+                                            seed + k * len(mv.cameras))
+        opt.zero_grad()
+        p.grad = torch.from_numpy(dmat.astype(np.float32))
+        if t is not None:
+            t.grad = torch.from_numpy(dtex.astype(np.float32))
+        opt.step()
+        with torch.no_grad():
+            p.copy_(torch.max(torch.min(p, hi), lo))
+            if t is not None:
+                t.clamp_(0.0, 1.0)
+        res.losses.append(loss)
+        if log and (k % 20 == 0 or k == steps - 1):
+            log(f"inverse step {k}: loss {loss:.6g}")
+    res.seconds = time.perf_counter() - t0
+    res.steps = steps
+    res.params = p.detach().numpy().copy()
+    res.texture = None if t is None else t.detach().numpy().copy()
+    return res
+
+
+def param_rmse(a: np.ndarray, b: np.ndarray, mask: np.ndarray | None = None, weights=None) -> float:
+    """RMSE of a - b over `mask`; with `weights` (e.g. |dL/dtheta| at the
+    start: the error along the directions the images see) a weighted RMSE."""
+    d = (np.asarray(a, np.float64) - np.asarray(b, np.float64)) ** 2
+    w = np.ones_like(d) if weights is None else np.abs(np.asarray(weights, np.float64))
+    if mask is not None:
+        d, w = d[mask], w[mask]
+    return float(np.sqrt((w * d).sum() / w.sum()))
+
+
+__all__ = ["MultiView", "InverseResult", "optimize", "orbit_views", "param_rmse", "material_params",
+           "PARAM_LO", "PARAM_HI"]

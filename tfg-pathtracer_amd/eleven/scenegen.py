@@ -342,8 +342,13 @@ def sky_hdr(w, h, sun_dir=(0.3, 0.6, 0.4), peak=1e3, sigma_deg=2.0):
 
 
 def heightfield(out_dir, cells=(1024, 512), extent=(51.2, 25.6), res=(1920, 1080), seed=0, n_mats=16,
-                tex_size=1024, lights=8, sky=(2048, 1024), aperture=0.05, bokeh=True):
-    """Heightfield scene (the 1,048,576-triangle workload at the default size)."""
+                tex_size=1024, lights=8, sky=(2048, 1024), aperture=0.05, bokeh=True, textured=None):
+    """Heightfield scene (the 1,048,576-triangle workload at the default size).
+    `textured`: how many materials (the first ones) carry their albedo map;
+    the others get a scalar albedo from a separate seeded stream (None: all
+    textured; the geometry, maps, lights and camera do not depend on it)."""
+    n_tex = n_mats if textured is None else textured
+    rng_kd = np.random.default_rng(seed + 7)
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     rng = np.random.default_rng(seed)
@@ -390,9 +395,12 @@ def heightfield(out_dir, cells=(1024, 512), extent=(51.2, 25.6), res=(1920, 1080
             img = np.clip(nz, 0, 1)[..., None] * rng.uniform(0.3, 1.0, 3)
         name = f"albedo_{k}.bmp"
         write_bmp(out / name, (np.clip(img, 0, 1) * 255).astype(np.uint8))
-        mats.append({"name": f"mat_{k}", "Kd": (0.8, 0.8, 0.8), "Ks": (0.5, 0.5, 0.5),
-                     "Pr": float(rng.uniform(0.2, 1.0)), "Pm": float(rng.uniform() < 0.2),
-                     "map_Kd": name})
+        mat = {"name": f"mat_{k}", "Kd": (0.8, 0.8, 0.8), "Ks": (0.5, 0.5, 0.5),
+               "Pr": float(rng.uniform(0.2, 1.0)), "Pm": float(rng.uniform() < 0.2), "map_Kd": name}
+        if k >= n_tex:
+            mat["Kd"] = tuple(float(x) for x in rng_kd.uniform(0.1, 0.9, 3))
+            del mat["map_Kd"]
+        mats.append(mat)
     ow.write(out / "scene.obj")
     write_mtl(out / "scene.mtl", mats)
     (out / "HDRI").mkdir(exist_ok=True)
