@@ -153,7 +153,9 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
     albedo) from a seeded start: parameters perturbed by N(0, 0.1^2), texture
     flat 0.5.  Targets: the true parameters and texture rendered at 64 spp.
     One gradient = all 8 views (forward + adjoint each); grads/s over the
-    Adam steps, parameter and texel RMSE before and after.  Gradients are
+    Adam steps, parameter and texel RMSE before and after (texels also as
+    32x32-texel block means: material 0's tile covers ~128x64 pixels of a
+    view, so single texels are not observable).  Gradients are
     decorrelated (eleven.inverse): each view is rendered twice per step,
     once for the residual and once for the paths the adjoint replays."""
     import eleven
@@ -176,6 +178,10 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
     mask = np.ones(truth.shape, bool)
     mask[0, :3] = False   # material 0's scalar albedo is replaced by its texture
 
+    def blocks(t, k=32):   # the texture at a resolution the 512x512 views resolve
+        h, w, c = t.shape
+        return t.reshape(h // k, k, w // k, k, c).mean(axis=(1, 3))
+
     out = {"metric": "inverse grads/s", "value": round(res_.grads_per_s, 3), "unit": "grads/s",
            "config": f"heightfield 1,048,576 tris, {res}x{res}, {views} views, {spp} spp forward + {spp} spp "
                      f"replay, depth {depth}, Adam lr 1e-2, 16 materials x 6 parameters + 1024x1024 albedo",
@@ -186,7 +192,10 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
            "param_rmse_gradient_weighted": {"init": round(inverse.param_rmse(p0, truth, mask, g0), 5),
                                             "final": round(inverse.param_rmse(res_.params, truth, mask, g0), 5)},
            "texel_rmse": {"init": round(inverse.param_rmse(t0, tex_truth), 5),
-                          "final": round(inverse.param_rmse(res_.texture, tex_truth), 5)}}
+                          "final": round(inverse.param_rmse(res_.texture, tex_truth), 5)},
+           "texture_rmse_32x32_block_means": {"init": round(inverse.param_rmse(blocks(t0), blocks(tex_truth)), 5),
+                                              "final": round(inverse.param_rmse(blocks(res_.texture),
+                                                                                blocks(tex_truth)), 5)}}
     ctx.close()
     return out
 
