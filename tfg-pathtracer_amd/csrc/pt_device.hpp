@@ -739,7 +739,9 @@ __device__ __forceinline__ TRay makeTRay(float ox, float oy, float oz, float dx,
 
 // Visits inner node `cur`: slab-tests its W children against [0, tcull],
 // pushes the hit children far-to-near and returns the nearest (or the next
-// stack entry if none is hit; -1 when the stack is empty).  Child encoding:
+// stack entry if none is hit; -1 when the stack is empty).  With UNSORTED
+// (any-hit queries, whose result does not depend on the visiting order) the
+// hit children are taken in slot order instead.  Child encoding:
 // >= 0 inner node, <= -2 leaf, -1 empty slot (its box is inverted, so it is
 // never hit).
 //
@@ -747,7 +749,7 @@ __device__ __forceinline__ TRay makeTRay(float ox, float oy, float oz, float dx,
 // the computed slab distances (b - o) * (1/d) carry only relative rounding
 // error (< 2^-21 with the reciprocal's); widening tfar by 2^-20 of its
 // magnitude makes the test conservative: no box the ray touches is culled.
-template <int W>
+template <int W, bool UNSORTED = false>
 __device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cur, const TRay& ray, float tcull,
                                          TravStack& st) {
     constexpr int D = W / 4;   // dwords of bytes per (axis, side)
@@ -797,6 +799,16 @@ __device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cu
         float tf = fminf(fminf((bx1 - ray.ox) * ray.ix, (by1 - ray.oy) * ray.iy), fminf((bz1 - ray.oz) * ray.iz, tcull));
         tf = fmaf(fabsf(tf), 1.0f / 1048576.0f, tf);
         keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+    }
+    if constexpr (UNSORTED) {
+        int next = -2;   // the lowest hit slot is visited next, the others are pushed
+#pragma unroll
+        for (int i = W - 1; i >= 0; i--) {
+            if (keys[i] == 0xFFFFFFFFu) continue;
+            if (next != -2) st.push(child[next]);
+            next = i;
+        }
+        return next == -2 ? st.pop() : child[next];
     }
     // ascending sort (tn >= 0, so float bits order as unsigned)
 #define EL_CSWAP(a, b)                        \
@@ -882,7 +894,7 @@ __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, flo
     while (cur != -1) {
         if (cur >= 0) {
             if (COUNT) cnt.nodes++;
-            cur = visitNode<W>(nodes, cur, ray, cullT<ANY>(r.t), st);
+            cur = visitNode<W, ANY>(nodes, cur, ray, cullT<ANY>(r.t), st);
         } else {
             bool done = testLeaf<ANY, COUNT>(s.tris, cur, ray, r, cnt);
             cur = done ? -1 : st.pop();
@@ -1006,7 +1018,8 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
             if (__all(!act || leaf != -1)) break;   // also when no lane can descend
             if (act) {
                 if (COUNT) cnt.nodes++;
-                cur = visitNode<W>(PER_RAY_BVH ? bvh.nodes : (const uint4*)s.nodes, cur, ray, cullT<ANY>(r.t), st);
+                cur = visitNode<W, ANY>(PER_RAY_BVH ? bvh.nodes : (const uint4*)s.nodes, cur, ray, cullT<ANY>(r.t),
+                                        st);
                 if (cur <= -2 && leaf == -1) {
                     leaf = cur;
                     cur = st.pop();
