@@ -90,7 +90,7 @@ struct el_ctx {
     std::vector<dev::DevTex> texs;
     std::vector<uint64_t> texF32;   // FP32-pool offset of each texture (el_set_texture)
 
-    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad, dEnvTab, dTexels8, dTexLut;
+    DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad, dEnvTab, dTexels8, dTexLut, dTangents;
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
     DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters;
     size_t wfCap = 0, wfQcap = 0;
@@ -322,21 +322,23 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     int rc = uploadBvh(c, c->width);
     if (rc) return rc;
 
-    // shading stream, indexed by original triangle
-    std::vector<float> shade((size_t)v->tri_count * dev::kShadeStride, 0.f);
+    // shading records and corner tangents, indexed by original triangle
+    std::vector<float> shade((size_t)v->tri_count * dev::kShadeFloats, 0.f);
+    std::vector<float> tans((size_t)v->tri_count * dev::kTanFloats, 0.f);
     for (uint32_t i = 0; i < v->tri_count; i++) {
-        float* S = &shade[(size_t)i * dev::kShadeStride];
+        float* S = &shade[(size_t)i * dev::kShadeFloats];
         std::memcpy(S, v->vertices + (size_t)i * 9, 9 * sizeof(float));
         if (v->normals) std::memcpy(S + 9, v->normals + (size_t)i * 9, 9 * sizeof(float));
         if (v->uvs) std::memcpy(S + 18, v->uvs + (size_t)i * 6, 6 * sizeof(float));
-        if (v->tangents) std::memcpy(S + 24, v->tangents + (size_t)i * 9, 9 * sizeof(float));
-        S[33] = v->tangent_sign ? v->tangent_sign[i] : 0.f;
         int32_t obj = v->object_id[i];
         int32_t mat = (obj >= 0 && (uint32_t)obj < v->mesh_count) ? v->mesh_material[obj] : 0;
         if (mat < 0 || (uint32_t)mat >= v->material_count) mat = 0;
-        std::memcpy(&S[34], &mat, 4);
+        std::memcpy(&S[24], &mat, 4);
+        S[25] = v->tangent_sign ? v->tangent_sign[i] : 0.f;
+        if (v->tangents) std::memcpy(&tans[(size_t)i * dev::kTanFloats], v->tangents + (size_t)i * 9, 9 * sizeof(float));
     }
     EL_HIP(c, upload(c->dShade, shade.data(), shade.size() * sizeof(float), c->stream));
+    EL_HIP(c, upload(c->dTangents, tans.data(), tans.size() * sizeof(float), c->stream));
 
     // textures: one FP32 pool, the environment map last.  A texture whose
     // every value is an entry of one of the loader's byte -> value tables
@@ -429,7 +431,8 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     c->dAov.release();
 
     DevScene& s = c->ds;
-    s.shade = c->dShade.as<float>();
+    s.shade = c->dShade.as<float4>();
+    s.tangents = c->dTangents.as<float4>();
     s.mats = c->dMats.as<dev::DevMaterial>();
     s.texs = c->dTexs.as<dev::DevTex>();
     s.texels = c->dTexels.as<float>();

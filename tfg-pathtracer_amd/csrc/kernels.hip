@@ -424,10 +424,9 @@ EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restric
         v3 an = mk(0, 0, 0), at = mk(0, 0, 0), ab = mk(0, 0, 0);
         paths++;
         v3 L = walkPath<W>(s, a, x, y, idx, a.sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
-            if (pv.depth == 0) {
+            if (pv.depth == 0 && aov) {
                 an = pv.sh->normal;
-                at = pv.sh->tangent;
-                ab = pv.sh->bitangent;
+                surfaceTangents(s, *pv.sh, at, ab);
             }
         });
         overflow |= st.overflow;
@@ -809,9 +808,11 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
             q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
             if (bounce == 0 && q.pathAov) {
+                v3 at, ab;
+                surfaceTangents(s, sh, at, ab);
                 q.pathAov[3 * (uint64_t)slot + 0] = make_float4(sh.normal.x, sh.normal.y, sh.normal.z, 0);
-                q.pathAov[3 * (uint64_t)slot + 1] = make_float4(sh.tangent.x, sh.tangent.y, sh.tangent.z, 0);
-                q.pathAov[3 * (uint64_t)slot + 2] = make_float4(sh.bitangent.x, sh.bitangent.y, sh.bitangent.z, 0);
+                q.pathAov[3 * (uint64_t)slot + 1] = make_float4(at.x, at.y, at.z, 0);
+                q.pathAov[3 * (uint64_t)slot + 2] = make_float4(ab.x, ab.y, ab.z, 0);
             }
             verts++;
             rays += 1 + (vo.hasP ? 1 : 0);

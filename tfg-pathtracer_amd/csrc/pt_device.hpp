@@ -134,15 +134,19 @@ struct DevCamera {
     int32_t bokeh;
 };
 
-// Shading stream, 36 floats per original triangle.
-constexpr int kShadeStride = 36;
-// [0..8] v0 v1 v2 | [9..17] n0 n1 n2 | [18..23] uv0 uv1 uv2 (x,y) |
-// [24..32] t0 t1 t2 | [33] tangentsSign | [34] material id (int bits)
+// Shading record per original triangle: 8 float4 = 128 B, one cache line
+// per hit: [0..8] v0 v1 v2 | [9..17] n0 n1 n2 | [18..23] uv0 uv1 uv2 (x,y) |
+// [24] material id (int bits) | [25] tangentsSign | [26..31] unused.
+// The corner tangents, needed only for normal maps and the tangent AOVs,
+// are a separate stream of 3 float4 per triangle: t0 t1 t2 | unused.
+constexpr int kShadeWords = 8, kShadeFloats = 4 * kShadeWords;
+constexpr int kTanWords = 3, kTanFloats = 4 * kTanWords;
 
 struct DevScene {
     const void* nodes;             // WideNode<W>[]
     const float4* tris;            // 3 float4 per leaf-order triangle
-    const float* shade;            // kShadeStride floats per original triangle
+    const float4* shade;           // kShadeWords per original triangle
+    const float4* tangents;        // kTanWords per original triangle
     const DevMaterial* mats;
     const DevTex* texs;
     const float* texels;           // pool of all textures (RGB FP32)
@@ -256,7 +260,7 @@ __device__ __forceinline__ void uniformCircleSampling(float u1, float u2, float 
 struct HitData {
     float metallic, roughness, clearcoatGloss, clearcoat, anisotropic, eta, transmission, specular,
         specularTint, sheenTint, subsurface, sheen;
-    v3 emission, albedo, normal, tangent, bitangent;
+    v3 emission, albedo, normal;
 };
 
 __device__ __forceinline__ void createBasis(v3 n, v3& t, v3& b) {   // Disney.hpp:41-45
@@ -515,26 +519,29 @@ __device__ __forceinline__ void cameraRay(const DevCamera& c, int x, int y, floa
 
 // --------------------------------------------------------------- hit data --
 struct SurfaceHit {
-    v3 position, normal, tangent, bitangent;
+    v3 position, normal;
     float tu, tv;
-    int32_t material;
+    int32_t material, tri;
+    float u, v, sign;   // barycentrics and tangentsSign, for surfaceTangents
 };
 
 // Tri::hit's attribute part (Tri.hpp:70-157) from the traversal's (t, u, v).
 __device__ __forceinline__ SurfaceHit surfaceAt(const DevScene& s, int tri, float t, float u, float v, v3 o, v3 d) {
-    const float* S = s.shade + (uint64_t)tri * kShadeStride;
+    const float4* R = s.shade + (uint64_t)tri * kShadeWords;
+    const float4 A = R[0], B = R[1], C = R[2], D = R[3], E = R[4], F = R[5], G = R[6];
     SurfaceHit h;
-    v3 v0 = ld3(S + 0), v1 = ld3(S + 3), v2 = ld3(S + 6);
-    v3 n0 = ld3(S + 9), n1 = ld3(S + 12), n2 = ld3(S + 15);
-    float uv0x = S[18], uv0y = S[19], uv1x = S[20], uv1y = S[21], uv2x = S[22], uv2y = S[23];
-    v3 t0 = ld3(S + 24), t1 = ld3(S + 27), t2 = ld3(S + 30);
-    float sign = S[33];
-    h.material = __float_as_int(S[34]);
+    v3 v0 = mk(A.x, A.y, A.z), v1 = mk(A.w, B.x, B.y), v2 = mk(B.z, B.w, C.x);
+    v3 n0 = mk(C.y, C.z, C.w), n1 = mk(D.x, D.y, D.z), n2 = mk(D.w, E.x, E.y);
+    float uv0x = E.z, uv0y = E.w, uv1x = F.x, uv1y = F.y, uv2x = F.z, uv2y = F.w;
+    h.material = __float_as_int(G.x);
+    h.sign = G.y;
+    h.tri = tri;
+    h.u = u;
+    h.v = v;
     h.tu = uv0x + (uv1x - uv0x) * u + (uv2x - uv0x) * v;
     h.tv = uv0y + (uv1y - uv0y) * u + (uv2y - uv0y) * v;
     v3 geom = o + d * t;
     v3 Ns = n0 + (n1 - n0) * u + (n2 - n0) * v;
-    v3 Ts = t0 + (t1 - t0) * u + (t2 - t0) * v;
     v3 p0 = geom - dot(geom - v0, n0) * n0;
     v3 p1 = geom - dot(geom - v1, n1) * n1;
     v3 p2 = geom - dot(geom - v2, n2) * n2;
@@ -542,9 +549,17 @@ __device__ __forceinline__ SurfaceHit surfaceAt(const DevScene& s, int tri, floa
     bool convex = dot(sp - geom, Ns) > 0.0f;
     h.position = convex ? sp : geom;
     h.normal = Ns;
-    h.tangent = Ts;
-    h.bitangent = sign * cross(Ns, Ts);
     return h;
+}
+
+// Tangent frame at a hit (Tri.hpp:78-79, 153): the interpolated corner
+// tangents (not renormalised) and bitangent = tangentsSign * N x T.
+__device__ __forceinline__ void surfaceTangents(const DevScene& s, const SurfaceHit& h, v3& T, v3& B) {
+    const float4* R = s.tangents + (uint64_t)h.tri * kTanWords;
+    const float4 a = R[0], b = R[1], c = R[2];
+    const v3 t0 = mk(a.x, a.y, a.z), t1 = mk(a.w, b.x, b.y), t2 = mk(b.z, b.w, c.x);
+    T = t0 + (t1 - t0) * h.u + (t2 - t0) * h.v;
+    B = h.sign * cross(h.normal, T);
 }
 
 // generateHitData (kernel.cu:54-119)
@@ -566,7 +581,9 @@ __device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& 
     } else {
         v3 c = texUV(P, s.texs[m.normalTex], h.tu, h.tv);
         v3 ln = c * 2.0f - mk(1, 1, 1);
-        d.normal = normalized(ln.x * h.tangent - ln.y * h.bitangent + ln.z * h.normal);
+        v3 T, B;
+        surfaceTangents(s, h, T, B);
+        d.normal = normalized(ln.x * T - ln.y * B + ln.z * h.normal);
     }
     d.clearcoatGloss = m.clearcoatGloss;
     d.clearcoat = m.clearcoat;
@@ -578,8 +595,6 @@ __device__ __forceinline__ HitData hitData(const DevScene& s, const SurfaceHit& 
     d.sheenTint = m.sheenTint;
     d.subsurface = m.subsurface;
     d.sheen = m.sheen;
-    d.tangent = h.tangent;
-    d.bitangent = h.bitangent;
     return d;
 }
 
