@@ -49,6 +49,11 @@ extern "C" {
                                  and sample counts are the same either way) */
 #define EL_FLAG_COUNT_TRAVERSAL 2u /* wavefront: tally node fetches and
                                       triangle tests of the closest-hit stage */
+#define EL_FLAG_RECORD_PATHS 8u /* keep each path's radiance (16 B per path)
+                                   so that el_adjoint replays every path once
+                                   instead of twice; needs a cleared
+                                   accumulation (the record covers this one
+                                   render)                                    */
 
 /* Passes, same numbering as the reference's `enum Passes` (kernel.h:7). */
 #define EL_PASS_BEAUTY     0
@@ -227,7 +232,9 @@ int  el_set_camera(el_ctx* ctx, const el_camera* camera);
  * el_read_passes.  Each path is replayed from its RNG counters, nothing of a
  * path is stored; a material whose albedo comes from a texture gets no
  * albedo gradient (its texels do, if selected).  With `device_ptrs` != 0 the
- * three arrays are device pointers. */
+ * three arrays are device pointers.  If that render used
+ * EL_FLAG_RECORD_PATHS, each path is walked once (the recorded radiance
+ * gives the radiance after each vertex); otherwise twice. */
 int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,
                 float* d_materials, float* d_texture, int device_ptrs);
 

@@ -169,3 +169,32 @@ def test_adjoint_error_paths(gpu_ctx, small):
     gpu_ctx.render(1, depth=17, seed=1)
     with pytest.raises(eleven.ElevenError):
         gpu_ctx.adjoint(np.zeros((H, W, 3), np.float32))          # deeper than the replay records
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [eleven.MODE_WAVEFRONT, eleven.MODE_PLAIN])
+def test_adjoint_from_recorded_paths_equals_two_walk_adjoint(gpu_ctx, small, mode):
+    """With EL_FLAG_RECORD_PATHS the adjoint takes each path's radiance from
+    the forward render instead of walking the path a first time: the same
+    gradients up to the FP32 summation order of the two radiance sums."""
+    print(f"[adjoint] recorded vs two-walk, mode {mode}", flush=True)
+    scene, _ = small
+    H, W = scene.camera.y_res, scene.camera.x_res
+    gpu_ctx.upload(scene)
+    w = np.random.default_rng(4).normal(size=(H, W, 3)).astype(np.float32)
+    gpu_ctx.clear()
+    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, mode=mode)
+    m1, t1 = gpu_ctx.adjoint(w, grad_texture=0)
+    r1 = gpu_ctx.stats().adjoint_rays
+    gpu_ctx.clear()
+    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, mode=mode, flags=eleven.FLAG_RECORD_PATHS)
+    m2, t2 = gpu_ctx.adjoint(w, grad_texture=0)
+    r2 = gpu_ctx.stats().adjoint_rays
+    assert r2 < r1                       # one walk per path instead of two
+    assert np.linalg.norm(m2 - m1) <= 1e-4 * np.linalg.norm(m1)
+    assert np.linalg.norm(t2 - t1) <= 1e-4 * np.linalg.norm(t1)
+    gpu_ctx.render(1, depth=DEPTH, seed=SEED, mode=mode)   # the record no longer covers the accumulation
+    m3, _ = gpu_ctx.adjoint(w, grad_texture=0)
+    assert np.abs(m3).max() > 0
+    with pytest.raises(eleven.ElevenError):
+        gpu_ctx.render(1, depth=DEPTH, seed=SEED, mode=mode, flags=eleven.FLAG_RECORD_PATHS)

@@ -102,6 +102,8 @@ struct el_ctx {
     uint64_t accSeed = 0;          // seed / depth / RNG scene word of the renders in the accumulation
     uint32_t accDepth = 0, accScene = 0;
     bool accMixed = false;         // renders since el_clear used different seeds or depths
+    DevBuf dRec;                   // per (sample, pixel) path radiance of a recorded render
+    bool recValid = false;         // dRec covers the whole accumulation
     uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
     DevBuf aImage, aMat, aTex;     // adjoint image, FP64 material and FP32 texel gradients
     el_stats stats{};
@@ -290,6 +292,7 @@ static int clearAccum(el_ctx* c) {
     if (c->dAov.p) EL_HIP(c, hipMemsetAsync(c->dAov.p, 0, c->dAov.n, c->stream));
     c->sampleBase = 0;
     c->accMixed = false;
+    c->recValid = false;
     c->totalRays = c->totalVerts = c->totalPaths = 0;
     EL_HIP(c, hipStreamSynchronize(c->stream));
     return EL_OK;
@@ -549,6 +552,13 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     a.maxDepth = depth;
     a.sample = c->sampleBase;
     a.cull = (p->flags & EL_FLAG_NO_CULL) ? 0u : 1u;
+    const bool record = (p->flags & EL_FLAG_RECORD_PATHS) != 0;
+    if (record) {
+        if (c->sampleBase != 0) return fail(c, EL_ERR_STATE, "EL_FLAG_RECORD_PATHS needs a cleared accumulation");
+        EL_HIP(c, c->dRec.ensure((size_t)p->spp * npx * sizeof(float4)));
+        a.rec = c->dRec.as<float4>();
+    }
+    c->recValid = false;
 
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
     if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
@@ -666,6 +676,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         c->accMixed = true;
     }
     c->sampleBase += p->spp;
+    c->recValid = record;
     c->totalRays += tally(h, kStatRays);
     c->totalVerts += tally(h, kStatVerts);
     c->totalPaths += tally(h, kStatPaths);
@@ -723,8 +734,9 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
     if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
     EL_HIP(c, hipEventRecord(e0, c->stream));
-    EL_HIP(c, launchAdjoint(lc, c->ds, a, c->sampleBase, c->aImage.as<float>(), c->dCounts.as<uint32_t>(), gtex,
-                            c->aMat.as<double>(), ntex ? c->aTex.as<float>() : nullptr, ctr));
+    EL_HIP(c, launchAdjoint(lc, c->ds, a, c->sampleBase, c->aImage.as<float>(), c->dCounts.as<uint32_t>(),
+                            c->recValid ? c->dRec.as<float4>() : nullptr, gtex, c->aMat.as<double>(),
+                            ntex ? c->aTex.as<float>() : nullptr, ctr));
     EL_HIP(c, hipEventRecord(e1, c->stream));
     std::vector<double> hm(M * 6);
     EL_HIP(c, hipMemcpyAsync(hm.data(), c->aMat.p, M * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1313,6 +1325,7 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
         c->accSeed = b->seed[i];
         c->accDepth = depth;
         c->accMixed = true;   // batch RNG streams carry the scene index: no el_adjoint on them
+        c->recValid = false;
     }
     b->stats = el_stats{};
     b->stats.rays = tally(h, kStatRays);

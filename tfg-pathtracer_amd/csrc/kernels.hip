@@ -430,6 +430,7 @@ EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restric
             }
         });
         overflow |= st.overflow;
+        if (a.rec) a.rec[(uint64_t)a.sample * W_ * H_ + idx] = make_float4(L.x, L.y, L.z, 1.f);
         L = clamp3(L);
         if (!anyNan(L)) {
             float4 acc = accum[idx];
@@ -469,9 +470,9 @@ constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
 
 template <int W>
 EL_KERNEL(EL_WPE_PLAIN) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
-                                       const uint32_t* __restrict__ counts, int gradTex, double* __restrict__ dmat,
-                                       float* __restrict__ dtex, int* gstack, uint32_t gthreads,
-                                       unsigned long long* counters) {
+                                       const uint32_t* __restrict__ counts, const float4* __restrict__ rec,
+                                       int gradTex, double* __restrict__ dmat, float* __restrict__ dtex, int* gstack,
+                                       uint32_t gthreads, unsigned long long* counters) {
     TravStack st = makeStack(s, gstack, gthreads);
     TraceCounters cnt{0, 0};
     const uint32_t W_ = s.cam.xRes, H_ = s.cam.yRes;
@@ -497,7 +498,13 @@ EL_KERNEL(EL_WPE_PLAIN) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, c
                 const uint32_t idx = W_ * (H_ - y - 1) + x;
                 const uint32_t n = counts[idx];
                 uint32_t vdummy = 0;
-                const v3 Lt = walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, vdummy, [](const PathVertex&) {});
+                v3 Lt;
+                if (rec) {
+                    const float4 r = rec[(uint64_t)sample * W_ * H_ + idx];
+                    Lt = mk(r.x, r.y, r.z);
+                } else {
+                    Lt = walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, vdummy, [](const PathVertex&) {});
+                }
                 paths++;
                 const v3 Lc = clamp3(Lt);
                 float ad[3] = {0, 0, 0};
@@ -908,6 +915,7 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
             if (Lf.w == 0.f) continue;   // padding pixel
             paths++;
             uint32_t idx = q.pathPix[slot];
+            if (a.rec) a.rec[(uint64_t)q.pathSample[slot] * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
             v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
             float4 acc = accum[idx];
@@ -1067,16 +1075,16 @@ hipError_t launchPlain(const LaunchCfg& c, const DevScene& s, const RenderArgs& 
 }
 
 hipError_t launchAdjoint(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, uint32_t spp,
-                         const float* dimage, const uint32_t* counts, int gradTex, double* dmat, float* dtex,
-                         unsigned long long* counters) {
+                         const float* dimage, const uint32_t* counts, const float4* rec, int gradTex, double* dmat,
+                         float* dtex, unsigned long long* counters) {
     uint64_t total = (uint64_t)((s.cam.xRes + 15) / 16) * ((s.cam.yRes + 15) / 16) * 256 * spp;
     uint32_t grid = gridFor(total, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
     if (s.width == 4)
-        hipLaunchKernelGGL(k_adjoint<4>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, gradTex,
+        hipLaunchKernelGGL(k_adjoint<4>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, rec, gradTex,
                            dmat, dtex, c.gstack, c.maxThreads, counters);
     else
-        hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, gradTex,
+        hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, rec, gradTex,
                            dmat, dtex, c.gstack, c.maxThreads, counters);
     return hipGetLastError();
 }

@@ -47,6 +47,7 @@ struct RenderArgs {
     uint32_t genCount;      // wavefront: work items in the generation
     uint32_t npad;          // wavefront: padded pixel count (8x8 tiles)
     uint32_t cull;          // stop zero-throughput paths that cannot turn NaN
+    float4* rec;            // per (sample, pixel): the path's radiance before the clamp, or null
 };
 
 struct WfState {
@@ -93,9 +94,11 @@ hipError_t launchTrace(const LaunchCfg& c, const dev::DevScene& s, const float* 
                        unsigned long long* counters);
 hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, float4* accum,
                        uint32_t* counts, float4* aov, unsigned long long* counters);
+// rec: the forward render's per-path radiance (EL_FLAG_RECORD_PATHS) or null
+// (then every path is walked once for its radiance, once for the gradient).
 hipError_t launchAdjoint(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, uint32_t spp,
-                         const float* dimage, const uint32_t* counts, int gradTex, double* dmat, float* dtex,
-                         unsigned long long* counters);
+                         const float* dimage, const uint32_t* counts, const float4* rec, int gradTex, double* dmat,
+                         float* dtex, unsigned long long* counters);
 // Batch (many scenes, scene id per queue entry): scenes = device array of
 // DevScene; segs = the generation's segment table (device).
 hipError_t launchBtCamera(const LaunchCfg& c, const dev::DevScene* scenes, const BatchSeg* segs, uint32_t nseg,

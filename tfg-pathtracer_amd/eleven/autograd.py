@@ -20,7 +20,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from . import MODE_WAVEFRONT, Context, SceneData, el_material
+from . import FLAG_RECORD_PATHS, MODE_WAVEFRONT, Context, SceneData, el_material
 
 PARAM_NAMES = ("albedo_r", "albedo_g", "albedo_b", "roughness", "metallic", "specular")
 
@@ -61,7 +61,8 @@ class Renderer:
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
         self.ctx.clear()
-        self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode)
+        # the recorded path radiance lets the backward replay each path once
+        self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode, flags=FLAG_RECORD_PATHS)
         return self.ctx.read()[0][..., :3].copy()
 
     def __call__(self, params: torch.Tensor, texture: torch.Tensor | None = None) -> torch.Tensor:

@@ -26,7 +26,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import MODE_WAVEFRONT, Context, SceneData, el_camera, make_camera
+from . import FLAG_RECORD_PATHS, MODE_WAVEFRONT, Context, SceneData, el_camera, make_camera
 from .autograd import material_params, with_params
 
 # seed offset of the render whose paths the adjoint replays (decorrelation)
@@ -66,9 +66,12 @@ class MultiView:
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
 
-    def render_view(self, v: int, seed: int, spp: int | None = None) -> np.ndarray:
+    def render_view(self, v: int, seed: int, spp: int | None = None, record: bool = False) -> np.ndarray:
+        """Mean image of view v; with `record`, the render keeps its path
+        radiance for a following el_adjoint (which then walks each path once)."""
         self.ctx.set_camera(self.cameras[v])
-        self.ctx.render(spp or self.spp, depth=self.depth, seed=seed, mode=self.mode)
+        self.ctx.render(spp or self.spp, depth=self.depth, seed=seed, mode=self.mode,
+                        flags=FLAG_RECORD_PATHS if record else 0)
         return self.ctx.read()[0][..., :3].copy()
 
     def images(self, params, texture=None, seed: int = 0, spp: int | None = None) -> list:
@@ -83,9 +86,9 @@ class MultiView:
         dmat = np.zeros_like(np.asarray(params, np.float32))
         dtex = None
         for v in range(len(self.cameras)):
-            img = self.render_view(v, seed + v)
+            img = self.render_view(v, seed + v, record=not decorrelate)
             if decorrelate:
-                self.render_view(v, seed + v + DECORRELATE_SEED)   # the paths the adjoint replays
+                self.render_view(v, seed + v + DECORRELATE_SEED, record=True)   # the paths the adjoint replays
             diff = img.astype(np.float64) - targets[v]
             loss += float(np.sum(diff * diff))
             dm, dt = self.ctx.adjoint((2.0 * diff).astype(np.float32),
