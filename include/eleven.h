@@ -163,8 +163,12 @@ typedef struct {
 typedef struct {
     int32_t  grad_texture; /* texture whose texels receive gradients, -1: none */
     uint32_t block;        /* threads per block, 0 -> default               */
-    uint32_t flags;        /* reserved, 0                                   */
+    uint32_t flags;        /* EL_ADJ_*                                      */
 } el_adjoint_params;
+#define EL_ADJ_ACCUMULATE 1u /* add to the gradients of the previous el_adjoint
+                                calls (same grad_texture) instead of starting
+                                from zero, e.g. to sum the views of one
+                                scene; the outputs receive the running sums */
 
 /* ---- host-side scene loading (SceneLoader.hpp:7 loadScene) ------------ */
 typedef struct el_scene el_scene;
@@ -232,7 +236,9 @@ int  el_set_camera(el_ctx* ctx, const el_camera* camera);
  * el_read_passes.  Each path is replayed from its RNG counters, nothing of a
  * path is stored; a material whose albedo comes from a texture gets no
  * albedo gradient (its texels do, if selected).  With `device_ptrs` != 0 the
- * three arrays are device pointers.  If that render used
+ * three arrays are device pointers.  d_materials and d_texture may be NULL:
+ * nothing is copied out, and the sums stay in the context for following
+ * EL_ADJ_ACCUMULATE calls.  If that render used
  * EL_FLAG_RECORD_PATHS, each path is walked once (the recorded radiance
  * gives the radiance after each vertex); otherwise twice. */
 int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,

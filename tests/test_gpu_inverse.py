@@ -7,7 +7,9 @@
    mode: residual and derivative from the same paths) equals central
    differences of the multi-view L2 loss with common random numbers (fixed
    seeds), within 1e-2 relative, for parameters sampled with seed 2.
-3. A short Adam run with decorrelated gradients from perturbed parameters
+3. EL_ADJ_ACCUMULATE sums the gradients of successive el_adjoint calls on
+   the device: the sum over views equals the sum of the separate calls.
+4. A short Adam run with decorrelated gradients from perturbed parameters
    lowers the loss and the parameter error weighted by the initial gradient
    magnitude (the error along the directions the images see: Adam moves
    every parameter by about lr per step, so ones the views hardly see drift
@@ -138,3 +140,30 @@ def test_short_inverse_run_reduces_parameter_error(gpu_ctx, scene_root):
           f"{res.grads_per_s:.1f} grads/s", flush=True)
     assert l1 < l0
     assert e1 < 0.8 * e0
+
+
+@pytest.mark.gpu
+def test_adjoint_accumulate_equals_sum_of_calls(gpu_ctx, scene_root):
+    print("[inverse] EL_ADJ_ACCUMULATE", flush=True)
+    scene = _scene(scene_root, "acc", res=(40, 32))
+    gpu_ctx.upload(scene)
+    views = _views(scene)
+    rng = np.random.default_rng(7)
+    parts = []
+    for v, cam in enumerate(views):
+        gpu_ctx.set_camera(cam)
+        gpu_ctx.render(4, depth=DEPTH, seed=20 + v, flags=eleven.FLAG_RECORD_PATHS)
+        w = rng.normal(size=(cam.y_res, cam.x_res, 3)).astype(np.float32)
+        parts.append(gpu_ctx.adjoint(w, grad_texture=0) + (w,))
+    total_m = sum(p[0].astype(np.float64) for p in parts)
+    total_t = sum(p[1].astype(np.float64) for p in parts)
+    for v, cam in enumerate(views):
+        gpu_ctx.set_camera(cam)
+        gpu_ctx.render(4, depth=DEPTH, seed=20 + v, flags=eleven.FLAG_RECORD_PATHS)
+        dm, dt = gpu_ctx.adjoint(parts[v][2], grad_texture=0, accumulate=v > 0, read=v == len(views) - 1)
+        if v < len(views) - 1:
+            assert dm is None and dt is None
+    assert np.linalg.norm(dm - total_m) <= 1e-5 * np.linalg.norm(total_m)
+    assert np.linalg.norm(dt - total_t) <= 1e-5 * np.linalg.norm(total_t)
+    with pytest.raises(eleven.ElevenError):   # another texture than the running sums'
+        gpu_ctx.adjoint(parts[0][2], grad_texture=-1, accumulate=True, read=False)

@@ -103,6 +103,8 @@ struct el_ctx {
     uint32_t accDepth = 0, accScene = 0;
     bool accMixed = false;         // renders since el_clear used different seeds or depths
     DevBuf dRec;                   // per (sample, pixel) path radiance of a recorded render
+    bool adjSumValid = false;      // aMat / aTex hold the sums of the el_adjoint calls so far
+    int32_t adjSumTex = -1;        // their grad_texture
     bool recValid = false;         // dRec covers the whole accumulation
     uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
     DevBuf aImage, aMat, aTex;     // adjoint image, FP64 material and FP32 texel gradients
@@ -316,6 +318,7 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
             return fail(c, EL_ERR_ARG, "texture " + std::to_string(i) + " is empty");
     EL_HIP(c, hipSetDevice(c->device));
     c->hasScene = false;
+    c->adjSumValid = false;
     c->ntri = v->tri_count;
     c->cam = v->camera;
 
@@ -697,7 +700,8 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
 
 extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* dimage, float* dmat, float* dtex,
                           int devp) {
-    if (!c || !p || !dimage || !dmat) return fail(c, EL_ERR_ARG, "null argument");
+    const bool accumulate = (p && (p->flags & EL_ADJ_ACCUMULATE) != 0);
+    if (!c || !p || !dimage) return fail(c, EL_ERR_ARG, "null argument");
     if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
     if (c->sampleBase == 0) return fail(c, EL_ERR_STATE, "nothing rendered since el_clear: no image to differentiate");
     if (c->accMixed) return fail(c, EL_ERR_STATE, "the accumulation mixes renders of different seeds or depths");
@@ -706,7 +710,8 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
         return fail(c, EL_ERR_ARG, "block must be a multiple of 64, at most 256");
     const int gtex = p->grad_texture;
     if (gtex >= (int)c->texs.size()) return fail(c, EL_ERR_ARG, "grad_texture out of range");
-    if (gtex >= 0 && !dtex) return fail(c, EL_ERR_ARG, "grad_texture set but no texture gradient buffer");
+    if (accumulate && (!c->adjSumValid || c->adjSumTex != gtex))
+        return fail(c, EL_ERR_STATE, "EL_ADJ_ACCUMULATE without a previous el_adjoint with the same grad_texture");
     EL_HIP(c, hipSetDevice(c->device));
     const size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
     const size_t M = c->mats.size();
@@ -714,11 +719,14 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     const hipMemcpyKind h2d = devp ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
     EL_HIP(c, c->aImage.ensure(npx * 3 * sizeof(float)));
     EL_HIP(c, hipMemcpyAsync(c->aImage.p, dimage, npx * 3 * sizeof(float), h2d, c->stream));
-    EL_HIP(c, c->aMat.ensure(std::max<size_t>(1, M) * 6 * sizeof(double)));
-    EL_HIP(c, hipMemsetAsync(c->aMat.p, 0, M * 6 * sizeof(double), c->stream));
-    if (ntex) {
-        EL_HIP(c, c->aTex.ensure(ntex * sizeof(float)));
-        EL_HIP(c, hipMemsetAsync(c->aTex.p, 0, ntex * sizeof(float), c->stream));
+    if (!accumulate) {
+        EL_HIP(c, c->aMat.ensure(std::max<size_t>(1, M) * 6 * sizeof(double)));
+        EL_HIP(c, hipMemsetAsync(c->aMat.p, 0, M * 6 * sizeof(double), c->stream));
+        if (ntex) {
+            EL_HIP(c, c->aTex.ensure(ntex * sizeof(float)));
+            EL_HIP(c, hipMemsetAsync(c->aTex.p, 0, ntex * sizeof(float), c->stream));
+        }
+        c->adjSumValid = false;
     }
     unsigned long long* ctr = c->dCounters.as<unsigned long long>();
     EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, c->stream));
@@ -747,12 +755,16 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     EL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
     std::vector<float> fm(M * 6);
     for (size_t i = 0; i < M * 6; i++) fm[i] = (float)hm[i];
-    if (devp)
-        EL_HIP(c, hipMemcpy(dmat, fm.data(), fm.size() * sizeof(float), hipMemcpyHostToDevice));
-    else
-        std::memcpy(dmat, fm.data(), fm.size() * sizeof(float));
-    if (ntex)
+    if (dmat) {
+        if (devp)
+            EL_HIP(c, hipMemcpy(dmat, fm.data(), fm.size() * sizeof(float), hipMemcpyHostToDevice));
+        else
+            std::memcpy(dmat, fm.data(), fm.size() * sizeof(float));
+    }
+    if (ntex && dtex)
         EL_HIP(c, hipMemcpy(dtex, c->aTex.p, ntex * sizeof(float), devp ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
+    c->adjSumValid = true;
+    c->adjSumTex = gtex;
     c->stats.adjoint_ms = ms;
     c->stats.adjoint_rays = tally(h, kStatRays);
     if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack overflow (gradients incomplete)");

@@ -21,6 +21,7 @@ LIB_PATH = PKG_DIR / "libeleven.so"
 EL_OK, EL_ERR_ARG, EL_ERR_IO, EL_ERR_HIP, EL_ERR_STATE, EL_ERR_RANGE, EL_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
 MODE_PLAIN, MODE_WAVEFRONT = 0, 1
 FLAG_AOV, FLAG_COUNT_TRAVERSAL, FLAG_NO_CULL, FLAG_RECORD_PATHS = 1, 2, 4, 8
+ADJ_ACCUMULATE = 1
 PASS_BEAUTY, PASS_DENOISE, PASS_NORMAL, PASS_TANGENT, PASS_BITANGENT = 0, 1, 2, 3, 4
 
 
@@ -387,22 +388,27 @@ class Context:
         arr = (el_material * len(mats))(*mats)
         self._check(lib().el_set_materials(self._h, arr, len(mats)))
 
-    def adjoint(self, dimage, grad_texture=-1, block=0):
+    def adjoint(self, dimage, grad_texture=-1, block=0, accumulate=False, read=True):
         """Path-replay adjoint of the accumulated image: gradients of
         sum(dimage * I) (dimage [H][W][3], the layout of read()) with respect to
         every material's (albedo r, g, b, roughness, metallic, specular) and the
         texels of texture `grad_texture`.  Returns (dmat [M][6], dtex [h][w][3]
-        or None)."""
+        or None).  With `accumulate` the gradients add to those of the previous
+        calls (EL_ADJ_ACCUMULATE) and the running sums are returned; with
+        read=False nothing is copied out and (None, None) is returned (the
+        sums stay on the device for following accumulating calls)."""
         cam = self.camera
         di = np.ascontiguousarray(dimage, np.float32).reshape(cam.y_res, cam.x_res, 3)
         m = len(self.scene.materials())
-        dmat = np.zeros((m, 6), np.float32)
-        dtex = None
-        if grad_texture >= 0:
-            t = self.scene.view.textures[grad_texture]
-            dtex = np.zeros((t.height, t.width, 3), np.float32)
-        p = el_adjoint_params(grad_texture, block, 0)
-        self._check(lib().el_adjoint(self._h, C.byref(p), di.ctypes.data, dmat.ctypes.data,
+        dmat = dtex = None
+        if read:
+            dmat = np.zeros((m, 6), np.float32)
+            if grad_texture >= 0:
+                t = self.scene.view.textures[grad_texture]
+                dtex = np.zeros((t.height, t.width, 3), np.float32)
+        p = el_adjoint_params(grad_texture, block, ADJ_ACCUMULATE if accumulate else 0)
+        self._check(lib().el_adjoint(self._h, C.byref(p), di.ctypes.data,
+                                     dmat.ctypes.data if dmat is not None else None,
                                      dtex.ctypes.data if dtex is not None else None, 0))
         return dmat, dtex
 

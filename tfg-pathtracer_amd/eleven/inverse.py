@@ -66,13 +66,14 @@ class MultiView:
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
 
-    def render_view(self, v: int, seed: int, spp: int | None = None, record: bool = False) -> np.ndarray:
-        """Mean image of view v; with `record`, the render keeps its path
-        radiance for a following el_adjoint (which then walks each path once)."""
+    def render_view(self, v: int, seed: int, spp: int | None = None, record: bool = False, read: bool = True):
+        """Mean image of view v (None with read=False); with `record`, the
+        render keeps its path radiance for a following el_adjoint (which then
+        walks each path once)."""
         self.ctx.set_camera(self.cameras[v])
         self.ctx.render(spp or self.spp, depth=self.depth, seed=seed, mode=self.mode,
                         flags=FLAG_RECORD_PATHS if record else 0)
-        return self.ctx.read()[0][..., :3].copy()
+        return self.ctx.read()[0][..., :3].copy() if read else None
 
     def images(self, params, texture=None, seed: int = 0, spp: int | None = None) -> list:
         self.set_params(params, texture)
@@ -83,19 +84,18 @@ class MultiView:
         dtexture [H][W][3] or None)."""
         self.set_params(params, texture)
         loss = 0.0
-        dmat = np.zeros_like(np.asarray(params, np.float32))
-        dtex = None
-        for v in range(len(self.cameras)):
+        nv = len(self.cameras)
+        dmat = dtex = None
+        for v in range(nv):
             img = self.render_view(v, seed + v, record=not decorrelate)
-            if decorrelate:
-                self.render_view(v, seed + v + DECORRELATE_SEED, record=True)   # the paths the adjoint replays
+            if decorrelate:   # the paths the adjoint replays
+                self.render_view(v, seed + v + DECORRELATE_SEED, record=True, read=False)
             diff = img.astype(np.float64) - targets[v]
             loss += float(np.sum(diff * diff))
-            dm, dt = self.ctx.adjoint((2.0 * diff).astype(np.float32),
-                                      grad_texture=self.grad_texture if texture is not None else -1)
-            dmat += dm
-            if texture is not None:
-                dtex = dt if dtex is None else dtex + dt
+            # gradients summed over the views on the device, read once
+            dmat, dtex = self.ctx.adjoint((2.0 * diff).astype(np.float32),
+                                          grad_texture=self.grad_texture if texture is not None else -1,
+                                          accumulate=v > 0, read=v == nv - 1)
         return loss, dmat, dtex
 
 
