@@ -628,9 +628,8 @@ EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
         uint32_t x, y;
         bool valid = workPixel(s, p, x, y);
         uint32_t idx = valid ? s.cam.xRes * (s.cam.yRes - y - 1) + x : 0xFFFFFFFFu;
-        q.pathT[slot] = make_float4(1, 1, 1, 0);
-        q.pathL[slot] = make_float4(0, 0, 0, valid ? 1.f : 0.f);   // w: path is live
-        q.pathPix[slot] = idx;
+        // throughput and radiance start at (1, 0): the first shade assumes them
+        q.pathPix[slot] = idx;   // 0xFFFFFFFF: padding
         q.pathSample[slot] = sample;
         if (q.pathAov) {
             q.pathAov[3 * (uint64_t)slot + 0] = make_float4(0, 0, 0, 0);
@@ -761,9 +760,9 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                     const float4 A = rayIn[2 * (uint64_t)e], B = rayIn[2 * (uint64_t)e + 1];
                     const uint32_t slot = __float_as_uint(A.w);
                     const DevScene& s = sc.get(__float_as_uint(B.w));
-                    const float4 Tf = q.pathT[slot];
+                    const float4 Tf = bounce == 0 ? make_float4(1, 1, 1, 0) : q.pathT[slot];
                     const v3 ev = envLookup(s, mk(B.x, B.y, B.z)) * mk(Tf.x, Tf.y, Tf.z);
-                    const float4 L = q.pathL[slot];
+                    const float4 L = bounce == 0 ? make_float4(0, 0, 0, 1) : q.pathL[slot];
                     q.pathL[slot] = make_float4(L.x + ev.x, L.y + ev.y, L.z + ev.z, L.w);
                 }
                 hit = tri >= 0;
@@ -799,7 +798,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             const DevScene& s = sc.get(sid);
             const int tri = __float_as_int(hr.x);
             v3 o = mk(A.x, A.y, A.z), d = mk(B.x, B.y, B.z);
-            float4 Tf = q.pathT[slot];
+            const float4 Tf = bounce == 0 ? make_float4(1, 1, 1, 0) : q.pathT[slot];
             SurfaceHit sh = surfaceAt(s, tri, hr.y, hr.z, hr.w, o, d);
             HitData hd = hitData(s, sh, s_lut);
             uint32_t k0 = a.k0, k1 = a.k1;
@@ -811,7 +810,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             uint4 rb = rng.block(2 + bounce);
             uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
             vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
-            float4 L = q.pathL[slot];
+            const float4 L = bounce == 0 ? make_float4(0, 0, 0, 1) : q.pathL[slot];
             q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
             q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
             if (bounce == 0 && q.pathAov) {
@@ -911,10 +910,10 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
     uint32_t paths = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < lim; j += stride) {
         for (uint32_t slot = j; slot < a.genCount; slot += a.npad) {
-            float4 Lf = q.pathL[slot];
-            if (Lf.w == 0.f) continue;   // padding pixel
-            paths++;
             uint32_t idx = q.pathPix[slot];
+            if (idx == 0xFFFFFFFFu) continue;   // padding pixel
+            float4 Lf = q.pathL[slot];
+            paths++;
             if (a.rec) a.rec[(uint64_t)q.pathSample[slot] * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
             v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
@@ -962,9 +961,7 @@ EL_KERNEL(1) void k_bt_camera(ManyScenes sc, const BatchSeg* __restrict__ segs, 
         uint32_t x, y;
         const bool valid = workPixel(s, p, x, y);
         const uint32_t idx = valid ? s.cam.xRes * (s.cam.yRes - y - 1) + x : 0xFFFFFFFFu;
-        q.pathT[slot] = make_float4(1, 1, 1, 0);
-        q.pathL[slot] = make_float4(0, 0, 0, valid ? 1.f : 0.f);
-        q.pathPix[slot] = idx;
+        q.pathPix[slot] = idx;   // 0xFFFFFFFF: padding; throughput and radiance: set by the first shade
         q.pathSample[slot] = sample;
         v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
         uint32_t quiet = 0xFFFFFFFFu;
@@ -996,10 +993,10 @@ EL_KERNEL(1) void k_bt_finalize(ManyScenes sc, const BatchSeg* __restrict__ segs
         const DevScene& s = sc.get(g.scene);
         const uint32_t end = g.slotStart + g.count;
         for (uint32_t slot = g.slotStart + (t - g.finStart); slot < end; slot += g.npad) {
-            const float4 Lf = q.pathL[slot];
-            if (Lf.w == 0.f) continue;   // padding pixel
-            paths++;
             const uint32_t idx = q.pathPix[slot];
+            if (idx == 0xFFFFFFFFu) continue;   // padding pixel
+            const float4 Lf = q.pathL[slot];
+            paths++;
             const v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
             const float4 acc = s.accum[idx];

@@ -55,9 +55,9 @@ struct WfState {
     float4* hits;           // (tri, t, u, v) per entry of the current queue
     float4* shadowH;        // (o, slot) (d, tmax) (contribution) per entry
     float4* shadowP;
-    float4* pathT;          // throughput per path slot
-    float4* pathL;          // radiance per path slot, w = live flag
-    uint32_t* pathPix;      // output pixel index per path slot
+    float4* pathT;          // throughput per path slot (from the first shade on)
+    float4* pathL;          // radiance per path slot (from the first shade on)
+    uint32_t* pathPix;      // output pixel index per path slot, 0xFFFFFFFF: padding
     uint32_t* pathSample;   // sample index per path slot
     uint32_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b)
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
