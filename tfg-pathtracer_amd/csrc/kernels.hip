@@ -720,6 +720,20 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
     blockFlush<4>(counters, slots, vals);
 }
 
+// Adds `c` to a path's radiance.  The first bounce writes it (the radiance
+// starts at zero there and is not stored before); later bounces skip the
+// read-modify-write when every component is zero, which leaves L bit for bit
+// as the addition would (x + 0 == x, x + -0 == x; a NaN is still added).
+__device__ __forceinline__ void addRadiance(float4* __restrict__ pathL, uint32_t slot, int bounce, v3 c) {
+    if (bounce == 0) {
+        pathL[slot] = make_float4(0.f + c.x, 0.f + c.y, 0.f + c.z, 1.f);   // 0 + -0 == +0, as a sum
+        return;
+    }
+    if (c.x == 0.f && c.y == 0.f && c.z == 0.f) return;
+    const float4 L = pathL[slot];
+    pathL[slot] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+}
+
 // Escaped rays (a texture lookup) and hits (full shading) arrive mixed in
 // the queue.  Each block classifies a chunk of blockDim entries, adds the
 // environment term of its escaped rays at once and pushes its hits onto an
@@ -762,8 +776,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                     const DevScene& s = sc.get(__float_as_uint(B.w));
                     const float4 Tf = bounce == 0 ? make_float4(1, 1, 1, 0) : q.pathT[slot];
                     const v3 ev = envLookup(s, mk(B.x, B.y, B.z)) * mk(Tf.x, Tf.y, Tf.z);
-                    const float4 L = bounce == 0 ? make_float4(0, 0, 0, 1) : q.pathL[slot];
-                    q.pathL[slot] = make_float4(L.x + ev.x, L.y + ev.y, L.z + ev.z, L.w);
+                    addRadiance(q.pathL, slot, bounce, ev);
                 }
                 hit = tri >= 0;
             }
@@ -810,9 +823,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             uint4 rb = rng.block(2 + bounce);
             uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
             vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
-            const float4 L = bounce == 0 ? make_float4(0, 0, 0, 1) : q.pathL[slot];
-            q.pathL[slot] = make_float4(L.x + vo.base.x, L.y + vo.base.y, L.z + vo.base.z, L.w);
-            q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
+            addRadiance(q.pathL, slot, bounce, vo.base);
             if (bounce == 0 && q.pathAov) {
                 v3 at, ab;
                 surfaceTangents(s, sh, at, ab);
@@ -826,6 +837,8 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
             wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
             cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
+            // only an extended path reads its throughput again
+            if (cont) q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
         }
         const bool want[3] = {cont, wantH, wantP};
         uint64_t at[3];
