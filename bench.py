@@ -26,6 +26,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 from pathlib import Path
@@ -59,6 +60,30 @@ def measured_traffic(kernel="k_wf_extend"):
         if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
             return int(parts[2])
     return None
+
+
+def copy_bandwidth(gib=2, reps=5):
+    """Achievable device bandwidth on this GPU: the better of a device-to-device
+    copy and an elementwise copy kernel over `gib` GiB (bytes read + written
+    per second), timed with events on torch's current stream."""
+    import torch
+    n = (gib << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    best = 0.0
+    for op in (lambda: y.copy_(x), lambda: torch.mul(x, 1.0, out=y)):
+        op()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op()
+        e1.record()
+        e1.synchronize()
+        best = max(best, 2 * n * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del x, y
+    torch.cuda.empty_cache()
+    return best
 
 
 def cpu_baseline(scene, budget_s=15.0, threads=None):
@@ -280,10 +305,17 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--inverse-steps", type=int, default=200)
+    ap.add_argument("--copy-bw", action="store_true",
+                    help="only measure the achievable copy bandwidth (JSON line); the roofline leg runs this "
+                         "in a child process started before this process touches the GPU")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (3 launches): the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings")
     args = ap.parse_args()
+
+    if args.copy_bw:
+        print(json.dumps({"copy_gbs": copy_bandwidth()}))
+        return
 
     if args.copy_calib:
         import torch
@@ -300,6 +332,17 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding of this path is "
               f"a design note, not a measured configuration.", file=sys.stderr)
         sys.exit(2)
+
+    # torch and the library each load a HIP runtime: the copy kernel runs in
+    # its own process, before this one initialises the GPU
+    copy_gbs = None
+    if args.mode == "wavefront" and not args.no_count:
+        r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--copy-bw"], capture_output=True, text=True,
+                           timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"copy bandwidth run failed: {r.stderr[-2000:]}")
+        copy_gbs = json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"]
+        log(f"copy kernel: {copy_gbs:.0f} GB/s")
 
     ensure_built()
     import eleven
@@ -383,6 +426,8 @@ def main():
         traffic = measured_traffic() if defaults and width == 4 else None
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+                           "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None,
                            "traffic_source": "profiles/hbm_traffic_heightfield_1080p_64spp.md" if traffic else None,
                            "kernel": "k_wf_extend (closest-hit traversal)",
                            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_frame_launches,
