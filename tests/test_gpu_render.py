@@ -219,3 +219,45 @@ def test_8bit_texture_copy_is_exact(gpu_ctx, scenes, kind):
         b, cb, rb, _ = gpu_ctx.read()
         assert np.array_equal(ca, cb) and ra == rb
         assert np.array_equal(a, b[..., :3].astype(np.float64))
+
+
+@pytest.mark.gpu
+def test_generations_split_equals_one_pass(gpu_ctx, scenes):
+    """A queue smaller than the frame splits it into generations whose
+    boundaries fall inside a sample pass (here 1000 paths over a 96x54 frame,
+    padded to 8x8 tiles); each pixel still adds its samples in sample order,
+    so the image is bit-identical to one generation."""
+    print("[render] generations vs one pass", flush=True)
+    s = scenes["terrain_bokeh"]
+    a, ca, ra, va, _ = _render(gpu_ctx, s, 4, 5, eleven.MODE_WAVEFRONT, depth=6)
+    b, cb, rb, vb, _ = _render(gpu_ctx, s, 4, 5, eleven.MODE_WAVEFRONT, depth=6)
+    assert np.array_equal(a, b) and np.array_equal(ca, cb)   # deterministic
+    gpu_ctx.upload(s)
+    gpu_ctx.render(4, depth=6, seed=5, queue_cap=1000)
+    c, cc, rc, vc = gpu_ctx.read()
+    assert np.array_equal(cc, ca) and rc == ra and vc == va
+    assert np.array_equal(c[..., :3].astype(np.float64), a)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("res", [(1, 1), (13, 7), (64, 8)])
+def test_empty_scene_sees_the_environment(gpu_ctx, res):
+    """No triangles: every camera ray escapes, so each sample's radiance is
+    the constant environment (0.5, up to the lookup's filter weights) and
+    every pixel's mean is 0.5; one BVH query per path, no vertices; plain and
+    wavefront agree exactly.  Ragged frames (not multiples of the 8x8 tile)
+    and a 1x1 frame included."""
+    print(f"[render] empty scene {res}", flush=True)
+    scene = eleven.SceneData.from_arrays(np.zeros((0, 9), np.float32), camera=dict(x_res=res[0], y_res=res[1]),
+                                         hdri=np.full((1, 1, 3), 0.5, np.float32))
+    out = []
+    for mode in (eleven.MODE_PLAIN, eleven.MODE_WAVEFRONT):
+        gpu_ctx.upload(scene)
+        gpu_ctx.render(3, depth=4, seed=1, mode=mode)
+        img, cnt, rays, verts = gpu_ctx.read()
+        assert img.shape[:2] == (res[1], res[0])
+        assert (cnt == 3).all()
+        assert rays == res[0] * res[1] * 3 and verts == 0
+        np.testing.assert_allclose(img[..., :3], 0.5, rtol=0, atol=1e-6)
+        out.append(img)
+    assert np.array_equal(out[0], out[1])
