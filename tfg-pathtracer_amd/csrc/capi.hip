@@ -451,7 +451,7 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     s.envBad = c->dEnvBad.as<uint32_t>();
     s.env = env;
     s.envRadianceSum = hs.radianceSum;
-    EL_HIP(c, c->dEnvTab.ensure((size_t)h.width * h.height * 3 * sizeof(float4)));
+    EL_HIP(c, c->dEnvTab.ensure((size_t)h.width * h.height * 2 * sizeof(float4)));
     s.envTab = c->dEnvTab.as<float4>();
     s.envLog2G = hs.log2G;
     s.nLights = (int32_t)v->light_count;

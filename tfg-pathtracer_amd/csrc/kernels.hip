@@ -1022,14 +1022,14 @@ EL_KERNEL(1) void k_bt_finalize(ManyScenes sc, const BatchSeg* __restrict__ segs
     blockFlush<1>(counters, slots, vals);
 }
 
-// envTab[3k .. 3k+2] = envSampleAt(k) for every environment pixel k < n.
+// envTab[2k], envTab[2k+1] = envSampleAt(k) for every environment pixel k < n
+// (its shadow direction is recomputed from the direction at sampling time).
 __global__ void __launch_bounds__(256) k_env_table(DevScene s, float4* tab, uint32_t n) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const EnvSample e = envSampleAt(s, (int)k);
-    tab[3 * (uint64_t)k] = make_float4(e.dir.x, e.dir.y, e.dir.z, e.pdf);
-    tab[3 * (uint64_t)k + 1] = make_float4(e.value.x, e.value.y, e.value.z, 0.f);
-    tab[3 * (uint64_t)k + 2] = make_float4(e.shadowDir.x, e.shadowDir.y, e.shadowDir.z, 0.f);
+    tab[2 * (uint64_t)k] = make_float4(e.dir.x, e.dir.y, e.dir.z, e.pdf);
+    tab[2 * (uint64_t)k + 1] = make_float4(e.value.x, e.value.y, e.value.z, 0.f);
 }
 
 }  // namespace dev

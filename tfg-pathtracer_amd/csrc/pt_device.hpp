@@ -157,7 +157,7 @@ struct DevScene {
     const int32_t* envGuide;       // guide table (2^envLog2G + 1 pairs)
     const int32_t* envMap;         // per interval: open-interval and tie results
     const uint32_t* envBad;        // 2^16-bit map: random-number buckets whose HDRI pdf may be 0 or inf
-    const float4* envTab;          // 3 per HDRI pixel k: envSampleAt(k) (direction + pdf, value, shadow direction)
+    const float4* envTab;          // 2 per HDRI pixel k: envSampleAt(k) (direction + pdf, value)
     DevTex env;                    // env texture (texels in the pool)
     float envRadianceSum;
     int32_t envLog2G;
@@ -642,12 +642,12 @@ __device__ __forceinline__ EnvSample envSampleAt(const DevScene& s, int k) {
 __device__ __forceinline__ EnvSample envSample(const DevScene& s, float r1) {
     // HDRI::sample(r1): exactly binarySearch(cdf, r1, W*H)
     const int k = hdriLookup(s.envCdf, s.envGuide, s.envMap, s.envLog2G, r1);
-    const float4 a = s.envTab[3 * k], b = s.envTab[3 * k + 1], c = s.envTab[3 * k + 2];
+    const float4 a = s.envTab[2 * k], b = s.envTab[2 * k + 1];
     EnvSample e;
     e.dir = mk(a.x, a.y, a.z);
     e.pdf = a.w;
     e.value = mk(b.x, b.y, b.z);
-    e.shadowDir = mk(c.x, c.y, c.z);
+    e.shadowDir = normalized(e.dir);   // as envSampleAt
     return e;
 }
 
