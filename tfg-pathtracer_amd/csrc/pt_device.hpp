@@ -764,11 +764,11 @@ __device__ __forceinline__ TRay makeTRay(float ox, float oy, float oz, float dx,
 // the computed slab distances (b - o) * (1/d) carry only relative rounding
 // error (< 2^-21 with the reciprocal's); widening tfar by 2^-20 of its
 // magnitude makes the test conservative: no box the ray touches is culled.
+// visitWords tests a node whose words are already loaded (the root, kept in
+// scalar registers by the wavefront traversal); visitNode loads them.
 template <int W, bool UNSORTED = false>
-__device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cur, const TRay& ray, float tcull,
-                                         TravStack& st) {
+__device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, float tcull, TravStack& st) {
     constexpr int D = W / 4;   // dwords of bytes per (axis, side)
-    const uint4* np = nodes + (uint64_t)cur * QNodeWords<W>::kWords;
     const uint4 h = np[0];
     uint32_t q[6 * D];
     int32_t child[W];
@@ -850,6 +850,17 @@ __device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cu
     for (int i = W - 1; i >= 1; i--)
         if (i < nh) st.push(child[keys[i] & 7u]);
     return child[keys[0] & 7u];
+}
+
+template <int W, bool UNSORTED = false>
+__device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cur, const TRay& ray, float tcull,
+                                         TravStack& st) {
+    constexpr int K = QNodeWords<W>::kWords;
+    const uint4* np = nodes + (uint64_t)cur * K;
+    uint4 w[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) w[k] = np[k];
+    return visitWords<W, UNSORTED>(w, ray, tcull, st);
 }
 
 // Tests the triangles of leaf `code` (<= -2) and updates the closest hit:
@@ -959,6 +970,20 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                                                uint32_t refill, Load&& load, Store&& store, TraceCounters& cnt,
                                                uint32_t& traced) {
     BvhPtr bvh{(const uint4*)s.nodes, s.tris};
+    // One BVH for the whole wave: its root node (visited first by every ray)
+    // is read once into scalar registers, and a new ray is tested against it
+    // as it is loaded, so no ray fetches the root.
+    constexpr int K = QNodeWords<W>::kWords;
+    uint4 root[K];
+    if constexpr (!PER_RAY_BVH) {
+        const uint4* rn = (const uint4*)s.nodes;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint4 v = rn[k];
+            root[k] = make_uint4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
+                                 __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
+        }
+    }
     const uint32_t lane = __lane_id();
     const unsigned long long ltMask = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint32_t cursor = ws.begin, end = ws.end;   // wave-uniform
@@ -1011,8 +1036,15 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                         r.u = r.v = 0.f;
                         st.sp = 0;
                         st.lo = 0;
-                        cur = 0;
                         leaf = -1;
+                        cur = 0;
+                        if constexpr (!PER_RAY_BVH) {
+                            cur = visitWords<W, ANY>(root, ray, cullT<ANY>(r.t), st);
+                            if (cur <= -2) {
+                                leaf = cur;
+                                cur = st.pop();
+                            }
+                        }
                         item = i;
                         live = true;
                     } else {
