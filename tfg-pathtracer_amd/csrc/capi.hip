@@ -519,6 +519,53 @@ static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
     return EL_OK;
 }
 
+// Generation size and wavefront state for `total` work items of a frame
+// with `npad` padded pixels: Q paths per generation (queueCap, or by default
+// as many as a quarter of the free device memory holds), rounded to whole
+// sample passes or 64-path groups; q points at the context's buffers.
+static int wfPlan(el_ctx* c, const LaunchCfg& lc, uint64_t total, uint64_t npad, uint64_t queueCap, bool aov,
+                  WfState& q, uint64_t& Q) {
+    // Default: as many paths in flight as a quarter of the free device
+    // memory holds (~216 B of queue and path state per path), so a
+    // 1080p x 64 spp frame runs as a single generation.
+    Q = queueCap;
+    if (!Q) {
+        size_t freeB = 0, totalB = 0;
+        EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
+        const uint64_t perPath = 6 * 32 + 44 + (aov ? 48 : 0);
+        Q = std::max<uint64_t>(1u << 20, freeB / 4 / perPath);
+    }
+    if (Q >= npad)
+        Q = (Q / npad) * npad;
+    else
+        Q = std::max<uint64_t>(64, (Q / 64) * 64);
+    if (Q > total) Q = total;
+    if (Q > 0x7fffffffull) return fail(c, EL_ERR_RANGE, "queue capacity above 2^31");
+    // Entries one XCD's blocks can append in one launch: at most
+    // ceil(Q/8) + maxThreads/8 + block (grid-stride over <= maxThreads,
+    // block <= kMaxBlock).
+    const uint64_t segSize = (Q + kXcds - 1) / kXcds + lc.maxThreads / kXcds + kMaxBlock;
+    const uint64_t qcap = std::max<uint64_t>(Q, kXcds * segSize);
+    int rc = ensureWf(c, Q, qcap, aov);
+    if (rc) return rc;
+    q.ray[0] = c->wRay0.as<float4>();
+    q.ray[1] = c->wRay1.as<float4>();
+    q.hits = c->wHits.as<float4>();
+    q.shadowH = c->wShH.as<float4>();
+    q.shadowP = c->wShP.as<float4>();
+    q.pathT = c->wT.as<float4>();
+    q.pathL = c->wL.as<float4>();
+    q.pathPix = c->wPix.as<uint32_t>();
+    q.pathSample = c->wSample.as<uint32_t>();
+    q.pathQuiet = c->wQuiet.as<uint32_t>();
+    q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
+    q.counters = c->wCounters.as<uint32_t>();
+    q.segSize = (uint32_t)segSize;
+    q.gstack = lc.gstack;
+    q.gthreads = lc.maxThreads;
+    return EL_OK;
+}
+
 extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     if (!c || !p) return fail(c, EL_ERR_ARG, "null argument");
     if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
@@ -579,45 +626,10 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         const uint32_t tilesX = (c->cam.x_res + 7) / 8, tilesY = (c->cam.y_res + 7) / 8;
         const uint64_t npad = (uint64_t)tilesX * tilesY * 64;
         uint64_t total = (uint64_t)p->spp * npad;
-        // Default: as many paths in flight as a quarter of the free device
-        // memory holds (~216 B of queue and path state per path), so a
-        // 1080p x 64 spp frame runs as a single generation.
-        uint64_t Q = p->queue_cap;
-        if (!Q) {
-            size_t freeB = 0, totalB = 0;
-            EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
-            const uint64_t perPath = 6 * 32 + 44 + (aov ? 48 : 0);
-            Q = std::max<uint64_t>(1u << 20, freeB / 4 / perPath);
-        }
-        if (Q >= npad)
-            Q = (Q / npad) * npad;
-        else
-            Q = std::max<uint64_t>(64, (Q / 64) * 64);
-        if (Q > total) Q = total;
-        if (Q > 0x7fffffffull) return fail(c, EL_ERR_RANGE, "queue capacity above 2^31");
-        // Entries one XCD's blocks can append in one launch: at most
-        // ceil(Q/8) + maxThreads/8 + block (grid-stride over <= maxThreads,
-        // block <= kMaxBlock).
-        const uint64_t segSize = (Q + kXcds - 1) / kXcds + lc.maxThreads / kXcds + kMaxBlock;
-        const uint64_t qcap = std::max<uint64_t>(Q, kXcds * segSize);
-        int rc = ensureWf(c, Q, qcap, aov);
-        if (rc) return rc;
+        uint64_t Q = 0;
         WfState q{};
-        q.ray[0] = c->wRay0.as<float4>();
-        q.ray[1] = c->wRay1.as<float4>();
-        q.hits = c->wHits.as<float4>();
-        q.shadowH = c->wShH.as<float4>();
-        q.shadowP = c->wShP.as<float4>();
-        q.pathT = c->wT.as<float4>();
-        q.pathL = c->wL.as<float4>();
-        q.pathPix = c->wPix.as<uint32_t>();
-        q.pathSample = c->wSample.as<uint32_t>();
-        q.pathQuiet = c->wQuiet.as<uint32_t>();
-        q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
-        q.counters = c->wCounters.as<uint32_t>();
-        q.segSize = (uint32_t)segSize;
-        q.gstack = lc.gstack;
-        q.gthreads = lc.maxThreads;
+        int rc = wfPlan(c, lc, total, npad, p->queue_cap, aov, q, Q);
+        if (rc) return rc;
         a.npad = (uint32_t)npad;
         size_t ev = 2;
         std::vector<std::pair<size_t, int>> timed;   // (event index, kind) kind 0 extend 1 shade 2 shadow

@@ -41,6 +41,9 @@ constexpr int kMaxWaves = kMaxBlock / 64;
 #ifndef EL_WPE_PLAIN
 #define EL_WPE_PLAIN 4
 #endif
+#ifndef EL_WPE_ADJ
+#define EL_WPE_ADJ 4
+#endif
 #define EL_KERNEL(wpe) __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(wpe)))
 
 __device__ __forceinline__ uint32_t waveSumU32(uint32_t v) {
@@ -450,6 +453,100 @@ EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restric
 }
 
 // ---------------------------------------------------------------- adjoint --
+// Gradient of one path vertex (the bracket in the comment of k_adjoint
+// below): the six parameters of the vertex's material in g (albedo zeroed
+// when it comes from a texture: those go to the texels of gradTex with FP32
+// atomics, through the bilinear weights when the texture filters).  d: incoming direction, T: throughput
+// before the vertex, La: radiance the path gathers after the vertex.
+// Returns the material index.
+__device__ __forceinline__ int vertexGradient(const DevScene& s, v3 d, v3 T, const SurfaceHit& sh,
+                                              const HitData& hd, const VertexOut& vo, bool visH, bool visP, v3 La,
+                                              const float ad[3], int gradTex, float* __restrict__ dtex, float g[6]) {
+    const DevTex* gt = gradTex >= 0 ? &s.texs[gradTex] : nullptr;
+    v3 dfs[6], dfh[6], dfl[6];
+    disneyEvalGrad(d, hd, vo.bdir, dfs);
+    disneyEvalGrad(d, hd, vo.hdir, dfh);
+    if (vo.hasP)
+        disneyEvalGrad(d, hd, vo.ldir, dfl);
+    else
+#pragma unroll
+        for (int q = 0; q < 6; q++) dfl[q] = mk(0, 0, 0);
+    const float cosS = fabsf(dot(vo.bdir, hd.normal));
+    const v3 cS = T * hd.emission * (vo.w3 * cosS / vo.bpdf);
+    const v3 cH = visH ? T * vo.hval * (vo.w1 * fabsf(dot(vo.hdir, hd.normal)) / vo.hpdf) : mk(0, 0, 0);
+    const v3 cP = (vo.hasP && visP) ? T * vo.lrad * (vo.w2 * fabsf(dot(vo.ldir, hd.normal)) / vo.ppdf)
+                                    : mk(0, 0, 0);
+    const v3 ratio = mk(vo.f.x != 0.f ? La.x / vo.f.x : 0.f, vo.f.y != 0.f ? La.y / vo.f.y : 0.f,
+                        vo.f.z != 0.f ? La.z / vo.f.z : 0.f);
+    const v3 kS = (cS + ratio) * mk(ad[0], ad[1], ad[2]);
+    const v3 kH = cH * mk(ad[0], ad[1], ad[2]);
+    const v3 kP = cP * mk(ad[0], ad[1], ad[2]);
+#pragma unroll
+    for (int q = 0; q < 6; q++) g[q] = dot(kS, dfs[q]) + dot(kH, dfh[q]) + dot(kP, dfl[q]);
+    const int mat = sh.material;
+    const DevMaterial& m = s.mats[mat];
+    if (m.albedoTex >= 0) {
+        if (gt && m.albedoTex == gradTex) {
+            const float tu = sh.tu, tv = sh.tv;
+            if (gt->filter == 1) {
+                const float xx = tu * gt->w, yy = tv * gt->h;
+                const float t1x = floorf(xx), t1y = floorf(yy);
+                const float fx = xx - t1x, fy = yy - t1y;
+                const int cx[4] = {(int)t1x, (int)t1x + 1, (int)t1x, (int)t1x + 1};
+                const int cy[4] = {(int)t1y, (int)t1y, (int)t1y + 1, (int)t1y + 1};
+                const float wg[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
+                for (int k4 = 0; k4 < 4; k4++) {
+                    const uint64_t ti = 3 * (uint64_t)texelIndex(*gt, cx[k4], cy[k4]);
+                    for (int q = 0; q < 3; q++) atomicAdd(dtex + ti + q, g[q] * wg[k4]);
+                }
+            } else {
+                const uint64_t ti = 3 * (uint64_t)texelIndex(*gt, (int)(tu * gt->w), (int)(tv * gt->h));
+                for (int q = 0; q < 3; q++) atomicAdd(dtex + ti + q, g[q]);
+            }
+        }
+        g[0] = g[1] = g[2] = 0.f;
+    }
+    const float e = 2.2f;
+    g[3] = m.roughnessTex < 0 ? g[3] * e * powf(m.roughness, e - 1.0f) : 0.f;
+    g[4] = m.metallicTex < 0 ? g[4] * e * powf(m.metallic, e - 1.0f) : 0.f;
+    return mat;
+}
+
+// Adjoint weight of one path sample: dLoss/dI of its pixel over the pixel's
+// accepted samples, per channel, 0 where the [0, 10] clamp is active or the
+// sample was rejected (NaN) or the pixel has none.
+__device__ __forceinline__ void adjointWeight(v3 Lt, const float* __restrict__ dimage, uint32_t idx, uint32_t n,
+                                              float ad[3]) {
+    ad[0] = ad[1] = ad[2] = 0.f;
+    if (!n || anyNan(clamp3(Lt))) return;
+    const float* di = dimage + 3 * (uint64_t)idx;
+    const float Lr[3] = {Lt.x, Lt.y, Lt.z};
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) ad[ch] = (Lr[ch] >= 0.f && Lr[ch] <= 10.f) ? di[ch] / (float)n : 0.f;
+}
+
+// Sums (material, g[6]) records across the wave, one FP64 atomic per
+// material and parameter (lanes with m < 0 hold none).  Every lane of the
+// wave must call it.
+__device__ __forceinline__ void waveAddGradients(int m, const float g[6], double* __restrict__ dmat) {
+    const uint32_t lane = __lane_id();
+    unsigned long long pending = __ballot(m >= 0);
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int m0 = __shfl(m, leader);
+        const bool mine = m == m0;
+        const unsigned long long grp = __ballot(mine);
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            float v = mine ? g[q] : 0.f;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+            if ((int)lane == leader && v != 0.f) atomicAdd(dmat + (uint64_t)m0 * 6 + q, (double)v);
+        }
+        pending &= ~grp;
+    }
+}
+
 // Path-replay backpropagation (Vicini, Speierer, Jakob, "Path Replay
 // Backpropagation", SIGGRAPH 2021) of the estimator above, with detached
 // sampling: the sampled directions, lobe choices, pdfs and MIS weights of a
@@ -469,7 +566,7 @@ EL_KERNEL(EL_WPE_PLAIN) void k_plain(DevScene s, RenderArgs a, float4* __restric
 constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
 
 template <int W>
-EL_KERNEL(EL_WPE_PLAIN) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
+EL_KERNEL(EL_WPE_ADJ) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
                                        const uint32_t* __restrict__ counts, const float4* __restrict__ rec,
                                        int gradTex, double* __restrict__ dmat, float* __restrict__ dtex, int* gstack,
                                        uint32_t gthreads, unsigned long long* counters) {
@@ -506,70 +603,13 @@ EL_KERNEL(EL_WPE_PLAIN) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, c
                     Lt = walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, vdummy, [](const PathVertex&) {});
                 }
                 paths++;
-                const v3 Lc = clamp3(Lt);
-                float ad[3] = {0, 0, 0};
-                if (n && !anyNan(Lc)) {
-                    const float* di = dimage + 3 * (uint64_t)idx;
-                    const float Lr[3] = {Lt.x, Lt.y, Lt.z};
-#pragma unroll
-                    for (int ch = 0; ch < 3; ch++) ad[ch] = (Lr[ch] >= 0.f && Lr[ch] <= 10.f) ? di[ch] / (float)n : 0.f;
-                }
+                float ad[3];
+                adjointWeight(Lt, dimage, idx, n, ad);
                 if (ad[0] != 0.f || ad[1] != 0.f || ad[2] != 0.f) {
-                    const DevTex* gt = gradTex >= 0 ? &s.texs[gradTex] : nullptr;
                     walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
-                        const VertexOut& vo = *pv.vo;
-                        const HitData& hd = *pv.hd;
-                        v3 dfs[6], dfh[6], dfl[6];
-                        disneyEvalGrad(pv.d, hd, vo.bdir, dfs);
-                        disneyEvalGrad(pv.d, hd, vo.hdir, dfh);
-                        if (vo.hasP)
-                            disneyEvalGrad(pv.d, hd, vo.ldir, dfl);
-                        else
-#pragma unroll
-                            for (int q = 0; q < 6; q++) dfl[q] = mk(0, 0, 0);
-                        const v3 La = Lt - pv.Lincl;
-                        const float cosS = fabsf(dot(vo.bdir, hd.normal));
-                        const v3 cS = pv.T * hd.emission * (vo.w3 * cosS / vo.bpdf);
-                        const v3 cH = pv.visH ? pv.T * vo.hval * (vo.w1 * fabsf(dot(vo.hdir, hd.normal)) / vo.hpdf)
-                                              : mk(0, 0, 0);
-                        const v3 cP = (vo.hasP && pv.visP)
-                                          ? pv.T * vo.lrad * (vo.w2 * fabsf(dot(vo.ldir, hd.normal)) / vo.ppdf)
-                                          : mk(0, 0, 0);
-                        const v3 ratio = mk(vo.f.x != 0.f ? La.x / vo.f.x : 0.f, vo.f.y != 0.f ? La.y / vo.f.y : 0.f,
-                                            vo.f.z != 0.f ? La.z / vo.f.z : 0.f);
-                        const v3 kS = (cS + ratio) * mk(ad[0], ad[1], ad[2]);
-                        const v3 kH = cH * mk(ad[0], ad[1], ad[2]);
-                        const v3 kP = cP * mk(ad[0], ad[1], ad[2]);
                         float g[6];
-#pragma unroll
-                        for (int q = 0; q < 6; q++) g[q] = dot(kS, dfs[q]) + dot(kH, dfh[q]) + dot(kP, dfl[q]);
-                        const int mat = pv.sh->material;
-                        const DevMaterial& m = s.mats[mat];
-                        if (m.albedoTex >= 0) {
-                            if (gt && m.albedoTex == gradTex) {
-                                const float tu = pv.sh->tu, tv = pv.sh->tv;
-                                float* D = dtex;
-                                if (gt->filter == 1) {
-                                    const float xx = tu * gt->w, yy = tv * gt->h;
-                                    const float t1x = floorf(xx), t1y = floorf(yy);
-                                    const float fx = xx - t1x, fy = yy - t1y;
-                                    const int cx[4] = {(int)t1x, (int)t1x + 1, (int)t1x, (int)t1x + 1};
-                                    const int cy[4] = {(int)t1y, (int)t1y, (int)t1y + 1, (int)t1y + 1};
-                                    const float wg[4] = {(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy};
-                                    for (int k4 = 0; k4 < 4; k4++) {
-                                        const uint64_t ti = 3 * (uint64_t)texelIndex(*gt, cx[k4], cy[k4]);
-                                        for (int q = 0; q < 3; q++) atomicAdd(D + ti + q, g[q] * wg[k4]);
-                                    }
-                                } else {
-                                    const uint64_t ti = 3 * (uint64_t)texelIndex(*gt, (int)(tu * gt->w), (int)(tv * gt->h));
-                                    for (int q = 0; q < 3; q++) atomicAdd(D + ti + q, g[q]);
-                                }
-                            }
-                            g[0] = g[1] = g[2] = 0.f;
-                        }
-                        const float e = 2.2f;
-                        g[3] = m.roughnessTex < 0 ? g[3] * e * powf(m.roughness, e - 1.0f) : 0.f;
-                        g[4] = m.metallicTex < 0 ? g[4] * e * powf(m.metallic, e - 1.0f) : 0.f;
+                        const int mat = vertexGradient(s, pv.d, pv.T, *pv.sh, *pv.hd, *pv.vo, pv.visH, pv.visP,
+                                                       Lt - pv.Lincl, ad, gradTex, dtex, g);
                         if (nrec < kAdjMaxDepth) {
                             recMat[nrec] = mat;
 #pragma unroll
@@ -582,24 +622,8 @@ EL_KERNEL(EL_WPE_PLAIN) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, c
         }
         overflow |= st.overflow;
         // converged: reduce the records per material across the wave
-        for (int r = 0; r < (int)a.maxDepth && r < kAdjMaxDepth; r++) {
-            const int m = r < nrec ? recMat[r] : -1;
-            unsigned long long pending = __ballot(m >= 0);
-            while (pending) {
-                const int leader = __ffsll((long long)pending) - 1;
-                const int m0 = __shfl(m, leader);
-                const bool mine = m == m0;
-                const unsigned long long grp = __ballot(mine);
-#pragma unroll
-                for (int q = 0; q < 6; q++) {
-                    float v = mine ? recG[r][q] : 0.f;
-#pragma unroll
-                    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-                    if ((int)lane == leader && v != 0.f) atomicAdd(dmat + (uint64_t)m0 * 6 + q, (double)v);
-                }
-                pending &= ~grp;
-            }
-        }
+        for (int r = 0; r < (int)a.maxDepth && r < kAdjMaxDepth; r++)
+            waveAddGradients(r < nrec ? recMat[r] : -1, recG[r < nrec ? r : 0], dmat);
     }
     const int slots[4] = {kStatRays, kStatVerts, kStatPaths, kStatOverflow};
     const uint32_t vals[4] = {rays, verts, paths, overflow};

@@ -86,7 +86,7 @@ struct LaunchCfg {
     int* gstack;
 };
 
-// The environment-sample table of s (3 float4 per HDRI pixel), from the
+// The environment-sample table of s (2 float4 per HDRI pixel), from the
 // scene's textures; s.envTab is not read.
 hipError_t launchEnvTable(hipStream_t stream, const dev::DevScene& s, float4* tab);
 hipError_t launchTrace(const LaunchCfg& c, const dev::DevScene& s, const float* rays, const float* tmax, uint32_t n,
