@@ -559,9 +559,10 @@ __device__ __forceinline__ void waveAddGradients(int m, const float g[6], double
 // with a = dLoss/dI_pixel / accepted samples (0 on clamped channels) and
 // L_after = L - (radiance up to and including the vertex).  Nothing of the
 // path is stored: memory is the adjoint image, the counts and the gradients.
-// Per-vertex records (material, 6 gradients) are kept in registers until
-// the path ends; the wave then reduces them per material (ballot + shuffle
-// sums) and one lane per material adds them with an FP64 atomic.  Texel
+// Per-vertex records (material, 6 gradients) are kept per lane (private
+// memory: they are indexed by the vertex number) until the path ends; the
+// wave then reduces them per material (ballot + shuffle sums) and one lane
+// per material adds them with an FP64 atomic.  Texel
 // gradients of the selected texture go straight to FP32 atomics.
 constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
 

@@ -222,7 +222,7 @@ class SceneData:
         mm = np.ascontiguousarray(mesh_material if mesh_material is not None else [0], np.int32)
         mats = materials if materials is not None else [make_material()]
         marr = (el_material * len(mats))(*mats)
-        texs = [make_texture(t) for t in textures]
+        texs = [t if isinstance(t, el_texture) else make_texture(t) for t in textures]   # arrays or make_texture()s
         tarr = (el_texture * max(1, len(texs)))(*texs)
         if hdri is None:
             hdri = np.full((1, 1, 3), 0.0, np.float32)
