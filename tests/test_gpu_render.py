@@ -261,3 +261,22 @@ def test_empty_scene_sees_the_environment(gpu_ctx, res):
         np.testing.assert_allclose(img[..., :3], 0.5, rtol=0, atol=1e-6)
         out.append(img)
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [eleven.MODE_WAVEFRONT, eleven.MODE_PLAIN])
+def test_bvh8_renders_the_bvh4_image(gpu_ctx, scenes, mode):
+    """The closest hit does not depend on the tree (min t, ties to the lower
+    index), so both widths trace the same paths: identical counts, tallies
+    and pixels."""
+    print(f"[render] BVH8 vs BVH4, mode {mode}", flush=True)
+    s = scenes["terrain_bokeh"]
+    out = []
+    for width in (4, 8):
+        gpu_ctx.upload(s)
+        gpu_ctx.render(4, depth=6, seed=21, mode=mode, bvh_width=width)
+        out.append(gpu_ctx.read())
+    (a, ca, ra, va), (b, cb, rb, vb) = out
+    assert gpu_ctx.stats().bvh_width == 8
+    assert np.array_equal(ca, cb) and ra == rb and va == vb
+    assert np.array_equal(a, b)
