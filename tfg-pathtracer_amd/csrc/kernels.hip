@@ -947,23 +947,32 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
     uint32_t lim = a.genCount < a.npad ? a.genCount : a.npad;
     uint32_t paths = 0;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < lim; j += stride) {
+        // every slot of this thread is the same pixel: its sums stay in
+        // registers (added in sample order) and are stored once
+        if (q.pathPix[j] == 0xFFFFFFFFu) continue;   // padding pixel
+        const uint32_t idx = q.pathPix[j];
+        float4 acc = accum[idx];
+        uint32_t n = counts[idx];
+        float4 A[3];
+        if (aov)
+            for (int k = 0; k < 3; k++) A[k] = aov[3 * (uint64_t)idx + k];
         for (uint32_t slot = j; slot < a.genCount; slot += a.npad) {
-            uint32_t idx = q.pathPix[slot];
-            if (idx == 0xFFFFFFFFu) continue;   // padding pixel
-            float4 Lf = q.pathL[slot];
+            const float4 Lf = q.pathL[slot];
             paths++;
             if (a.rec) a.rec[(uint64_t)q.pathSample[slot] * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
-            v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
+            const v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
-            float4 acc = accum[idx];
-            accum[idx] = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
-            counts[idx] += 1;
+            acc = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
+            n += 1;
             if (aov) {
-                float4* A = aov + 3 * (uint64_t)idx;
                 const float4* P = q.pathAov + 3 * (uint64_t)slot;
                 for (int k = 0; k < 3; k++) A[k] = make_float4(A[k].x + P[k].x, A[k].y + P[k].y, A[k].z + P[k].z, 0);
             }
         }
+        accum[idx] = acc;
+        counts[idx] = n;
+        if (aov)
+            for (int k = 0; k < 3; k++) aov[3 * (uint64_t)idx + k] = A[k];
     }
     const int slots[1] = {kStatPaths};
     const uint32_t vals[1] = {paths};
@@ -1030,17 +1039,23 @@ EL_KERNEL(1) void k_bt_finalize(ManyScenes sc, const BatchSeg* __restrict__ segs
         const BatchSeg g = segs[findSeg(segs, nseg, t, true)];
         const DevScene& s = sc.get(g.scene);
         const uint32_t end = g.slotStart + g.count;
-        for (uint32_t slot = g.slotStart + (t - g.finStart); slot < end; slot += g.npad) {
-            const uint32_t idx = q.pathPix[slot];
-            if (idx == 0xFFFFFFFFu) continue;   // padding pixel
+        const uint32_t first = g.slotStart + (t - g.finStart);
+        // every slot of this thread is the same pixel of the scene: its sums
+        // stay in registers (added in sample order) and are stored once
+        const uint32_t idx = q.pathPix[first];
+        if (idx == 0xFFFFFFFFu) continue;   // padding pixel
+        float4 acc = s.accum[idx];
+        uint32_t n = s.counts[idx];
+        for (uint32_t slot = first; slot < end; slot += g.npad) {
             const float4 Lf = q.pathL[slot];
             paths++;
             const v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
-            const float4 acc = s.accum[idx];
-            s.accum[idx] = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
-            s.counts[idx] += 1;
+            acc = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
+            n += 1;
         }
+        s.accum[idx] = acc;
+        s.counts[idx] = n;
     }
     const int slots[1] = {kStatPaths};
     const uint32_t vals[1] = {paths};
