@@ -626,6 +626,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         const uint32_t tilesX = (c->cam.x_res + 7) / 8, tilesY = (c->cam.y_res + 7) / 8;
         const uint64_t npad = (uint64_t)tilesX * tilesY * 64;
         uint64_t total = (uint64_t)p->spp * npad;
+        if (npad >= (1ull << 31)) return fail(c, EL_ERR_RANGE, "frame too large for the wavefront renderer");
         uint64_t Q = 0;
         WfState q{};
         int rc = wfPlan(c, lc, total, npad, p->queue_cap, aov, q, Q);
@@ -634,7 +635,8 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         size_t ev = 2;
         std::vector<std::pair<size_t, int>> timed;   // (event index, kind) kind 0 extend 1 shade 2 shadow
         for (uint64_t start = 0; start < total; start += Q) {
-            a.genStart = start;
+            a.genPixel = (uint32_t)(start % npad);
+            a.genSample = (uint32_t)(start / npad);
             a.genCount = (uint32_t)std::min<uint64_t>(Q, total - start);
             EL_HIP(c, launchWfCamera(lc, c->ds, a, q));
             for (uint32_t b = 0; b < depth; b++) {

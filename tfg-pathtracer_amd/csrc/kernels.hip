@@ -642,20 +642,26 @@ __device__ __forceinline__ bool workPixel(const DevScene& s, uint32_t p, uint32_
     return x < s.cam.xRes && y < s.cam.yRes;
 }
 
+// Path slot of a single-scene generation -> output pixel index
+// (0xFFFFFFFF: padding) and sample index; (x, y) the kernel pixel.  The
+// camera, shade and finalize kernels recompute it instead of storing it.
+// genPixel + slot < 2^32: both are below 2^31 (checked on the host).
+__device__ __forceinline__ uint32_t slotPixel(const DevScene& s, const RenderArgs& a, uint32_t slot,
+                                              uint32_t& sample, uint32_t& x, uint32_t& y) {
+    const uint32_t u = a.genPixel + slot;
+    sample = a.sample + a.genSample + u / a.npad;
+    return workPixel(s, u % a.npad, x, y) ? s.cam.xRes * (s.cam.yRes - y - 1) + x : 0xFFFFFFFFu;
+}
+
 // Camera rays go to queue 0 at index = path slot (no compaction); padding
 // slots get a zero direction, which the extend kernel reports as dead.
 EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.genCount; slot += stride) {
-        uint64_t w = a.genStart + slot;
-        uint32_t p = (uint32_t)(w % a.npad);
-        uint32_t sample = a.sample + (uint32_t)(w / a.npad);
-        uint32_t x, y;
-        bool valid = workPixel(s, p, x, y);
-        uint32_t idx = valid ? s.cam.xRes * (s.cam.yRes - y - 1) + x : 0xFFFFFFFFu;
+        uint32_t sample, x, y;
+        const uint32_t idx = slotPixel(s, a, slot, sample, x, y);
+        const bool valid = idx != 0xFFFFFFFFu;
         // throughput and radiance start at (1, 0): the first shade assumes them
-        q.pathPix[slot] = idx;   // 0xFFFFFFFF: padding
-        q.pathSample[slot] = sample;
         if (q.pathAov) {
             q.pathAov[3 * (uint64_t)slot + 0] = make_float4(0, 0, 0, 0);
             q.pathAov[3 * (uint64_t)slot + 1] = make_float4(0, 0, 0, 0);
@@ -844,7 +850,15 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                 k0 = s.seedLo;
                 k1 = s.seedHi;
             }
-            PathRng rng{k0, k1, q.pathPix[slot], q.pathSample[slot], sid};
+            uint32_t pix, smp;
+            if constexpr (P::kMany) {
+                pix = q.pathPix[slot];
+                smp = q.pathSample[slot];
+            } else {
+                uint32_t px, py;
+                pix = slotPixel(s, a, slot, smp, px, py);
+            }
+            PathRng rng{k0, k1, pix, smp, sid};
             uint4 rb = rng.block(2 + bounce);
             uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
             vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
@@ -949,17 +963,18 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < lim; j += stride) {
         // every slot of this thread is the same pixel: its sums stay in
         // registers (added in sample order) and are stored once
-        if (q.pathPix[j] == 0xFFFFFFFFu) continue;   // padding pixel
-        const uint32_t idx = q.pathPix[j];
+        uint32_t sample, x, y;
+        const uint32_t idx = slotPixel(s, a, j, sample, x, y);
+        if (idx == 0xFFFFFFFFu) continue;   // padding pixel
         float4 acc = accum[idx];
         uint32_t n = counts[idx];
         float4 A[3];
         if (aov)
             for (int k = 0; k < 3; k++) A[k] = aov[3 * (uint64_t)idx + k];
-        for (uint32_t slot = j; slot < a.genCount; slot += a.npad) {
+        for (uint32_t slot = j; slot < a.genCount; slot += a.npad, sample++) {
             const float4 Lf = q.pathL[slot];
             paths++;
-            if (a.rec) a.rec[(uint64_t)q.pathSample[slot] * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
+            if (a.rec) a.rec[(uint64_t)sample * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
             const v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
             acc = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);

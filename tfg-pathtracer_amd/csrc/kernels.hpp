@@ -43,7 +43,8 @@ struct RenderArgs {
     uint32_t scene;         // scene id (counter word 3)
     uint32_t maxDepth;
     uint32_t bounce;        // wavefront: current bounce
-    uint64_t genStart;      // wavefront: first work item of the generation
+    uint32_t genPixel;      // wavefront: first work item of the generation, as padded pixel
+    uint32_t genSample;     //   and sample offset (item = genSample * npad + genPixel)
     uint32_t genCount;      // wavefront: work items in the generation
     uint32_t npad;          // wavefront: padded pixel count (8x8 tiles)
     uint32_t cull;          // stop zero-throughput paths that cannot turn NaN
@@ -57,8 +58,8 @@ struct WfState {
     float4* shadowP;
     float4* pathT;          // throughput per path slot (from the first shade on)
     float4* pathL;          // radiance per path slot (from the first shade on)
-    uint32_t* pathPix;      // output pixel index per path slot, 0xFFFFFFFF: padding
-    uint32_t* pathSample;   // sample index per path slot
+    uint32_t* pathPix;      // batches: output pixel index per path slot, 0xFFFFFFFF: padding
+    uint32_t* pathSample;   // batches: sample index per path slot (one scene: slotPixel)
     uint32_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b)
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
     uint32_t* counters;     // kQCounters: per-XCD segment counts of the four queues
