@@ -169,6 +169,9 @@ typedef struct {
                                 calls (same grad_texture) instead of starting
                                 from zero, e.g. to sum the views of one
                                 scene; the outputs receive the running sums */
+#define EL_ADJ_DEVICE_RESIDUAL 2u /* d_image is not read (may be NULL): the
+                                adjoint weights are the residual that the last
+                                el_residual_l2 left on the device */
 
 /* ---- host-side scene loading (SceneLoader.hpp:7 loadScene) ------------ */
 typedef struct el_scene el_scene;
@@ -243,6 +246,17 @@ int  el_set_camera(el_ctx* ctx, const el_camera* camera);
  * gives the radiance after each vertex); otherwise twice. */
 int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,
                 float* d_materials, float* d_texture, int device_ptrs);
+
+/* L2 residual of the accumulated image on the device (no counterpart in the
+ * reference; for inverse rendering): with I the mean image (the rgb of
+ * el_read_passes) and target [H][W][3] in host memory, keeps
+ * r = 2 (I - target) (computed in FP64, stored as FP32) for a following
+ * el_adjoint with EL_ADJ_DEVICE_RESIDUAL and writes sum (I - target)^2 to
+ * *loss (may be NULL).  The image never leaves the device.  The residual
+ * stays until the next el_residual_l2, el_set_camera or scene upload, so it
+ * may be taken from one render and the adjoint run on another (the
+ * decorrelated multi-view gradient). */
+int  el_residual_l2(el_ctx* ctx, const float* target, double* loss);
 
 /* Closest hit for n rays: tri = original triangle index or -1, t = hit
  * distance along the (caller-normalised) direction, uv = barycentrics

@@ -167,3 +167,35 @@ def test_adjoint_accumulate_equals_sum_of_calls(gpu_ctx, scene_root):
     assert np.linalg.norm(dt - total_t) <= 1e-5 * np.linalg.norm(total_t)
     with pytest.raises(eleven.ElevenError):   # another texture than the running sums'
         gpu_ctx.adjoint(parts[0][2], grad_texture=-1, accumulate=True, read=False)
+
+
+@pytest.mark.gpu
+def test_device_residual_equals_host_residual(gpu_ctx, scene_root):
+    """el_residual_l2 + EL_ADJ_DEVICE_RESIDUAL give the gradients of an
+    adjoint fed 2 (I - target) computed on the host from read() in FP64 (the
+    same FP32 residual; the gradients differ only by the order of the
+    atomic sums) and the same loss;
+    the residual survives el_clear and a second render (the decorrelated
+    step), and an adjoint asking for it without one fails."""
+    print("[inverse] device residual", flush=True)
+    scene = _scene(scene_root, "res", res=(40, 32))
+    gpu_ctx.upload(scene)
+    cam = _views(scene)[0]
+    gpu_ctx.set_camera(cam)
+    with pytest.raises(eleven.ElevenError):   # no residual yet
+        gpu_ctx.adjoint(None, grad_texture=0)
+    target = np.random.default_rng(5).uniform(0, 1, size=(cam.y_res, cam.x_res, 3)).astype(np.float32)
+    gpu_ctx.render(4, depth=DEPTH, seed=31)
+    img = gpu_ctx.read()[0][..., :3]
+    diff = img.astype(np.float64) - target
+    loss_dev = gpu_ctx.residual_l2(target)
+    assert abs(loss_dev - float(np.sum(diff * diff))) <= 1e-9 * float(np.sum(diff * diff))
+    gpu_ctx.clear()
+    gpu_ctx.render(4, depth=DEPTH, seed=32, flags=eleven.FLAG_RECORD_PATHS)
+    dm_dev, dt_dev = gpu_ctx.adjoint(None, grad_texture=0)
+    dm_host, dt_host = gpu_ctx.adjoint((2.0 * diff).astype(np.float32), grad_texture=0)
+    assert np.linalg.norm(dm_dev - dm_host) <= 1e-6 * np.linalg.norm(dm_host)
+    assert np.linalg.norm(dt_dev - dt_host) <= 1e-6 * np.linalg.norm(dt_host)
+    assert np.abs(dm_host).max() > 0
+    with pytest.raises(eleven.ElevenError):   # the host image replaced the residual
+        gpu_ctx.adjoint(None, grad_texture=0)

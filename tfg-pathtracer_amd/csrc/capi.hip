@@ -108,6 +108,8 @@ struct el_ctx {
     bool recValid = false;         // dRec covers the whole accumulation
     uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
     DevBuf aImage, aMat, aTex;     // adjoint image, FP64 material and FP32 texel gradients
+    DevBuf aTarget, aLoss;         // el_residual_l2: target image, FP64 loss
+    size_t residualPx = 0;         // aImage holds el_residual_l2's residual for this many pixels (0: none)
     el_stats stats{};
     bool aovOn = false;
 
@@ -319,6 +321,7 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     EL_HIP(c, hipSetDevice(c->device));
     c->hasScene = false;
     c->adjSumValid = false;
+    c->residualPx = 0;
     c->ntri = v->tri_count;
     c->cam = v->camera;
 
@@ -712,10 +715,32 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     return EL_OK;
 }
 
+extern "C" int el_residual_l2(el_ctx* c, const float* target, double* loss) {
+    if (!c || !target) return fail(c, EL_ERR_ARG, "null argument");
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (c->sampleBase == 0) return fail(c, EL_ERR_STATE, "nothing rendered since el_clear");
+    EL_HIP(c, hipSetDevice(c->device));
+    const size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
+    EL_HIP(c, c->aImage.ensure(npx * 3 * sizeof(float)));
+    EL_HIP(c, c->aTarget.ensure(npx * 3 * sizeof(float)));
+    EL_HIP(c, c->aLoss.ensure(sizeof(double)));
+    EL_HIP(c, hipMemcpyAsync(c->aTarget.p, target, npx * 3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    EL_HIP(c, hipMemsetAsync(c->aLoss.p, 0, sizeof(double), c->stream));
+    EL_HIP(c, launchResidualL2(c->stream, c->dAccum.as<float4>(), c->dCounts.as<uint32_t>(), c->aTarget.as<float>(),
+                               (uint32_t)npx, c->aImage.as<float>(), c->aLoss.as<double>()));
+    double h = 0.0;
+    EL_HIP(c, hipMemcpyAsync(&h, c->aLoss.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = h;
+    c->residualPx = npx;
+    return EL_OK;
+}
+
 extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* dimage, float* dmat, float* dtex,
                           int devp) {
     const bool accumulate = (p && (p->flags & EL_ADJ_ACCUMULATE) != 0);
-    if (!c || !p || !dimage) return fail(c, EL_ERR_ARG, "null argument");
+    const bool devResidual = (p && (p->flags & EL_ADJ_DEVICE_RESIDUAL) != 0);
+    if (!c || !p || (!dimage && !devResidual)) return fail(c, EL_ERR_ARG, "null argument");
     if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
     if (c->sampleBase == 0) return fail(c, EL_ERR_STATE, "nothing rendered since el_clear: no image to differentiate");
     if (c->accMixed) return fail(c, EL_ERR_STATE, "the accumulation mixes renders of different seeds or depths");
@@ -731,8 +756,14 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     const size_t M = c->mats.size();
     const size_t ntex = gtex >= 0 ? (size_t)c->texs[gtex].w * c->texs[gtex].h * 3 : 0;
     const hipMemcpyKind h2d = devp ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-    EL_HIP(c, c->aImage.ensure(npx * 3 * sizeof(float)));
-    EL_HIP(c, hipMemcpyAsync(c->aImage.p, dimage, npx * 3 * sizeof(float), h2d, c->stream));
+    if (devResidual) {
+        if (c->residualPx != npx)
+            return fail(c, EL_ERR_STATE, "EL_ADJ_DEVICE_RESIDUAL without an el_residual_l2 at this resolution");
+    } else {
+        EL_HIP(c, c->aImage.ensure(npx * 3 * sizeof(float)));
+        EL_HIP(c, hipMemcpyAsync(c->aImage.p, dimage, npx * 3 * sizeof(float), h2d, c->stream));
+        c->residualPx = 0;
+    }
     if (!accumulate) {
         EL_HIP(c, c->aMat.ensure(std::max<size_t>(1, M) * 6 * sizeof(double)));
         EL_HIP(c, hipMemsetAsync(c->aMat.p, 0, M * 6 * sizeof(double), c->stream));

@@ -1087,6 +1087,37 @@ __global__ void __launch_bounds__(256) k_env_table(DevScene s, float4* tab, uint
     tab[2 * (uint64_t)k + 1] = make_float4(e.value.x, e.value.y, e.value.z, 0.f);
 }
 
+// The L2 residual of the mean image (accum / counts, as el_read_passes
+// divides) against a target, and the block sums of its squares.
+__global__ void __launch_bounds__(256) k_residual_l2(const float4* __restrict__ accum,
+                                                     const uint32_t* __restrict__ counts,
+                                                     const float* __restrict__ target, uint32_t npx,
+                                                     float* __restrict__ r, double* loss) {
+    __shared__ double s_part[256 / 64];
+    double part = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npx; i += gridDim.x * blockDim.x) {
+        const float4 acc = accum[i];
+        const uint32_t n = counts[i];
+        const float fn = (float)n;
+        const float I[3] = {n ? acc.x / fn : 0.f, n ? acc.y / fn : 0.f, n ? acc.z / fn : 0.f};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double d = (double)I[k] - (double)target[3 * (uint64_t)i + k];
+            r[3 * (uint64_t)i + k] = (float)(2.0 * d);
+            part += d * d;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
+    if (__lane_id() == 0) s_part[threadIdx.x >> 6] = part;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = 0.0;
+        for (uint32_t w = 0; w < (blockDim.x >> 6); w++) b += s_part[w];
+        atomicAdd(loss, b);
+    }
+}
+
 }  // namespace dev
 
 // ------------------------------------------------------------ host launch --
@@ -1151,6 +1182,13 @@ hipError_t launchAdjoint(const LaunchCfg& c, const DevScene& s, const RenderArgs
     else
         hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, rec, gradTex,
                            dmat, dtex, c.gstack, c.maxThreads, counters);
+    return hipGetLastError();
+}
+
+hipError_t launchResidualL2(hipStream_t stream, const float4* accum, const uint32_t* counts, const float* target,
+                            uint32_t npx, float* r, double* loss) {
+    const uint32_t grid = gridFor(npx, 256, 4096);
+    hipLaunchKernelGGL(k_residual_l2, dim3(grid), dim3(256), 0, stream, accum, counts, target, npx, r, loss);
     return hipGetLastError();
 }
 

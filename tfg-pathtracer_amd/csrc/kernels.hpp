@@ -97,6 +97,10 @@ hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderA
                        uint32_t* counts, float4* aov, unsigned long long* counters);
 // rec: the forward render's per-path radiance (EL_FLAG_RECORD_PATHS) or null
 // (then every path is walked once for its radiance, once for the gradient).
+// r = 2 (mean image - target) per pixel and channel (FP64 arithmetic, FP32
+// store); *loss += sum (mean image - target)^2 (the caller zeroes it).
+hipError_t launchResidualL2(hipStream_t stream, const float4* accum, const uint32_t* counts, const float* target,
+                            uint32_t npx, float* r, double* loss);
 hipError_t launchAdjoint(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, uint32_t spp,
                          const float* dimage, const uint32_t* counts, const float4* rec, int gradTex, double* dmat,
                          float* dtex, unsigned long long* counters);

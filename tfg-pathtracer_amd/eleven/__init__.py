@@ -21,7 +21,7 @@ LIB_PATH = PKG_DIR / "libeleven.so"
 EL_OK, EL_ERR_ARG, EL_ERR_IO, EL_ERR_HIP, EL_ERR_STATE, EL_ERR_RANGE, EL_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
 MODE_PLAIN, MODE_WAVEFRONT = 0, 1
 FLAG_AOV, FLAG_COUNT_TRAVERSAL, FLAG_NO_CULL, FLAG_RECORD_PATHS = 1, 2, 4, 8
-ADJ_ACCUMULATE = 1
+ADJ_ACCUMULATE, ADJ_DEVICE_RESIDUAL = 1, 2
 PASS_BEAUTY, PASS_DENOISE, PASS_NORMAL, PASS_TANGENT, PASS_BITANGENT = 0, 1, 2, 3, 4
 
 
@@ -123,6 +123,7 @@ _SIGS = {
     "el_set_texture": (C.c_int, [_P, C.c_int32, _P]),
     "el_set_camera": (C.c_int, [_P, _P]),
     "el_adjoint": (C.c_int, [_P, C.POINTER(el_adjoint_params), _P, _P, _P, C.c_int]),
+    "el_residual_l2": (C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     "el_batch_create": (C.c_int, [C.POINTER(_P), C.c_int]),
     "el_batch_destroy": (None, [_P]),
     "el_batch_last_error": (C.c_char_p, [_P]),
@@ -388,6 +389,16 @@ class Context:
         arr = (el_material * len(mats))(*mats)
         self._check(lib().el_set_materials(self._h, arr, len(mats)))
 
+    def residual_l2(self, target) -> float:
+        """el_residual_l2: keeps 2 (I - target) on the device for an
+        adjoint(None, ...) and returns sum((I - target)^2) (I = the mean image
+        of read(), target [H][W][3])."""
+        cam = self.camera
+        t = np.ascontiguousarray(target, np.float32).reshape(cam.y_res, cam.x_res, 3)
+        loss = C.c_double()
+        self._check(lib().el_residual_l2(self._h, t.ctypes.data, C.byref(loss)))
+        return loss.value
+
     def adjoint(self, dimage, grad_texture=-1, block=0, accumulate=False, read=True):
         """Path-replay adjoint of the accumulated image: gradients of
         sum(dimage * I) (dimage [H][W][3], the layout of read()) with respect to
@@ -396,9 +407,11 @@ class Context:
         or None).  With `accumulate` the gradients add to those of the previous
         calls (EL_ADJ_ACCUMULATE) and the running sums are returned; with
         read=False nothing is copied out and (None, None) is returned (the
-        sums stay on the device for following accumulating calls)."""
+        sums stay on the device for following accumulating calls).  With
+        dimage None the residual of the last residual_l2 is used
+        (EL_ADJ_DEVICE_RESIDUAL)."""
         cam = self.camera
-        di = np.ascontiguousarray(dimage, np.float32).reshape(cam.y_res, cam.x_res, 3)
+        di = None if dimage is None else np.ascontiguousarray(dimage, np.float32).reshape(cam.y_res, cam.x_res, 3)
         m = len(self.scene.materials())
         dmat = dtex = None
         if read:
@@ -406,8 +419,9 @@ class Context:
             if grad_texture >= 0:
                 t = self.scene.view.textures[grad_texture]
                 dtex = np.zeros((t.height, t.width, 3), np.float32)
-        p = el_adjoint_params(grad_texture, block, ADJ_ACCUMULATE if accumulate else 0)
-        self._check(lib().el_adjoint(self._h, C.byref(p), di.ctypes.data,
+        flags = (ADJ_ACCUMULATE if accumulate else 0) | (ADJ_DEVICE_RESIDUAL if di is None else 0)
+        p = el_adjoint_params(grad_texture, block, flags)
+        self._check(lib().el_adjoint(self._h, C.byref(p), di.ctypes.data if di is not None else None,
                                      dmat.ctypes.data if dmat is not None else None,
                                      dtex.ctypes.data if dtex is not None else None, 0))
         return dmat, dtex

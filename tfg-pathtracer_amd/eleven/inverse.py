@@ -87,14 +87,15 @@ class MultiView:
         nv = len(self.cameras)
         dmat = dtex = None
         for v in range(nv):
-            img = self.render_view(v, seed + v, record=not decorrelate)
-            if decorrelate:   # the paths the adjoint replays
-                self.render_view(v, seed + v + DECORRELATE_SEED, record=True, read=False)
-            diff = img.astype(np.float64) - targets[v]
-            loss += float(np.sum(diff * diff))
+            # the residual 2 (I - target) is formed on the device and stays there
+            self.render_view(v, seed + v, record=not decorrelate, read=False)
+            loss += self.ctx.residual_l2(targets[v])
+            if decorrelate:   # the paths the adjoint replays (same view: the residual stays valid)
+                self.ctx.clear()
+                self.ctx.render(self.spp, depth=self.depth, seed=seed + v + DECORRELATE_SEED, mode=self.mode,
+                                flags=FLAG_RECORD_PATHS)
             # gradients summed over the views on the device, read once
-            dmat, dtex = self.ctx.adjoint((2.0 * diff).astype(np.float32),
-                                          grad_texture=self.grad_texture if texture is not None else -1,
+            dmat, dtex = self.ctx.adjoint(None, grad_texture=self.grad_texture if texture is not None else -1,
                                           accumulate=v > 0, read=v == nv - 1)
         return loss, dmat, dtex
 
