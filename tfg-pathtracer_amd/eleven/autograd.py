@@ -55,15 +55,16 @@ class Renderer:
         self.grad_texture, self.mode = grad_texture, mode
         self.base = scene.materials()
 
-    def render(self, params: np.ndarray, texture: np.ndarray | None = None) -> np.ndarray:
-        """Forward only (numpy in, numpy out)."""
+    def render(self, params: np.ndarray, texture: np.ndarray | None = None, read: bool = True) -> np.ndarray | None:
+        """Forward only (numpy in, numpy out; read=False leaves the image on
+        the device, e.g. for Context.residual_l2)."""
         self.ctx.set_materials(with_params(self.base, params))
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
         self.ctx.clear()
         # the recorded path radiance lets the backward replay each path once
         self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode, flags=FLAG_RECORD_PATHS)
-        return self.ctx.read()[0][..., :3].copy()
+        return self.ctx.read()[0][..., :3].copy() if read else None
 
     def __call__(self, params: torch.Tensor, texture: torch.Tensor | None = None) -> torch.Tensor:
         return _RenderFn.apply(params, texture, self)
