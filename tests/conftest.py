@@ -24,7 +24,7 @@ def has_gpu():
 def gpu_ctx():
     import eleven
     if eleven.device_count() == 0:
-        pytest.fail("no HIP device visible: GPU tests must run on a GPU box")
+        pytest.fail("no HIP device visible: the tests marked gpu need an MI355X")
     ctx = eleven.Context(0)
     yield ctx
     ctx.close()
