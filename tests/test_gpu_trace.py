@@ -30,18 +30,18 @@ def _check(tri_g, t_g, tri_o, t_o, label):
 
 
 @pytest.fixture(scope="module")
-def c0(scene_root):
-    d = scenegen.sphere_field(scene_root / "c0")
+def field64(scene_root):
+    d = scenegen.sphere_field(scene_root / "field64")
     return eleven.SceneData.load(d)
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("width", [4, 8])
-def test_closest_random_rays_vs_oracle(gpu_ctx, c0, width):
+def test_closest_random_rays_vs_oracle(gpu_ctx, field64, width):
     print(f"[trace] sphere field, 2^20 random rays, BVH{width}", flush=True)
-    gpu_ctx.upload(c0)
+    gpu_ctx.upload(field64)
     gpu_ctx.render(0, bvh_width=width)          # switch width without rendering
-    verts = c0.vertices()
+    verts = field64.vertices()
     rays = raygen.random_rays(verts, 1 << 20, seed=1)
     tri_g, t_g, _ = gpu_ctx.trace_closest(rays)
     ref = orc.RefBVH(verts)
@@ -61,11 +61,11 @@ def test_closest_random_rays_vs_oracle(gpu_ctx, c0, width):
 
 
 @pytest.mark.gpu
-def test_closest_camera_and_tie_rays(gpu_ctx, c0):
+def test_closest_camera_and_tie_rays(gpu_ctx, field64):
     print("[trace] sphere field camera rays (256x256x16) + 4096 tie rays", flush=True)
-    gpu_ctx.upload(c0)
-    verts = c0.vertices()
-    cam = raygen.camera_rays(c0.camera, spp=16, seed=2)
+    gpu_ctx.upload(field64)
+    verts = field64.vertices()
+    cam = raygen.camera_rays(field64.camera, spp=16, seed=2)
     ties = raygen.tie_rays(verts, 4096, seed=3)
     rays = np.concatenate([cam, ties])
     tri_g, t_g, uv_g = gpu_ctx.trace_closest(rays)
@@ -83,10 +83,10 @@ def test_closest_camera_and_tie_rays(gpu_ctx, c0):
 
 
 @pytest.mark.gpu
-def test_any_hit_matches_closest(gpu_ctx, c0):
+def test_any_hit_matches_closest(gpu_ctx, field64):
     print("[trace] any-hit vs closest", flush=True)
-    gpu_ctx.upload(c0)
-    verts = c0.vertices()
+    gpu_ctx.upload(field64)
+    verts = field64.vertices()
     rays = raygen.random_rays(verts, 1 << 16, seed=4)
     tri, t, _ = gpu_ctx.trace_closest(rays)
     rng = np.random.default_rng(5)

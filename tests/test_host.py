@@ -168,7 +168,7 @@ def test_loader_errors_are_clean(tmp_path, broken, code):
 
 
 def test_sphere_field_workload_size(tmp_path):
-    s = eleven.SceneData.load(scenegen.sphere_field(tmp_path / "c0"))
+    s = eleven.SceneData.load(scenegen.sphere_field(tmp_path / "sphere_field"))
     assert s.tri_count == 61442 and s.view.material_count == 65 and s.view.light_count == 2
 
 
