@@ -77,22 +77,26 @@ __device__ __forceinline__ void blockFlush(unsigned long long* counters, const i
 // Chunked work of a traversal launch over n queue items (counter set `kind`):
 // chunks of kChunkRays, smaller (down to 64) when the queue would not give
 // every wave of the grid eight of them.
+template <bool WIDE>
 __device__ __forceinline__ WorkSource chunkSource(uint32_t* qcounters, int kind, uint32_t n) {
     WorkSource ws;
     ws.begin = ws.end = 0;
-    ws.ctr = qcounters + kQChunk + kind * kXcds * kChunkStride;
+    ws.ctr = qcounters + kQChunk + kind * kChunkShares * kChunkStride;
     ws.n = n;
     const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t fit = (uint32_t)(((uint64_t)n / ((uint64_t)waves * 8)) & ~63ull);
     ws.chunk = fit < 64 ? 64u : (fit > kChunkRays ? kChunkRays : fit);
     ws.stride = kChunkStride;
-    ws.parts = kXcds;
-    ws.home = xcdOf();
+    ws.chunks = (uint32_t)(((uint64_t)n + ws.chunk - 1) / ws.chunk);
+    static_assert(kChunkShares == (int)kWorkShares, "one chunk share per lane");
+    ws.parts = WIDE ? kChunkShares : kXcds;
+    ws.home = blockIdx.x % ws.parts;
     return ws;
 }
-// Zeroes the chunk counters of `kind` (threads 0..kXcds-1 of block 0 call).
+// Zeroes the chunk counters of `kind` (threads 0..kChunkShares-1 of block 0).
 __device__ __forceinline__ void resetChunks(uint32_t* qcounters, int kind) {
-    if (blockIdx.x == 0 && threadIdx.x < kXcds) qcounters[kQChunk + (kind * kXcds + threadIdx.x) * kChunkStride] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kChunkShares)
+        qcounters[kQChunk + (kind * kChunkShares + threadIdx.x) * kChunkStride] = 0;
 }
 
 // Dense index i over an XCD-segmented queue -> entry index.
@@ -723,7 +727,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
     resetChunks(q.counters, kChunkShH);   // for the shadow launches that follow
     resetChunks(q.counters, kChunkShP);
     uint32_t rays = 0;
-    const WorkSource ws = chunkSource(q.counters, kChunkExt, n);
+    const WorkSource ws = chunkSource<P::kMany>(q.counters, kChunkExt, n);
     const uint32_t segSize = q.segSize;
     auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
                     BvhPtr& bvh) {
@@ -919,7 +923,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
     qm.load(countIn);
     const uint32_t n = qm.total();
     uint32_t traced = 0;
-    const WorkSource ws = chunkSource(q.counters, sh == q.shadowH ? kChunkShH : kChunkShP, n);
+    const WorkSource ws = chunkSource<P::kMany>(q.counters, sh == q.shadowH ? kChunkShH : kChunkShP, n);
     const uint32_t segSize = q.segSize;
     auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
                     BvhPtr& bvh) {

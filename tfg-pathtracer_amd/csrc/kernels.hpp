@@ -32,9 +32,12 @@ constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, 
 // Queue counters: kXcds per queue.
 constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadowH = 2 * kXcds, kQShadowP = 3 * kXcds;
 // Work-chunk counters of the traversal launches (extend, HDRI shadow, light
-// shadow), one per XCD share, each on its own 128-byte line.
+// shadow), each on its own 128-byte line: a batch launch draws from
+// kChunkShares shares (share x first by the blocks b with b % 64 == x), a
+// single-scene launch from the first kXcds (one per XCD).
+constexpr int kChunkShares = 64;
 constexpr int kChunkStride = 32, kChunkExt = 0, kChunkShH = 1, kChunkShP = 2;
-constexpr int kQChunk = 4 * kXcds, kQCounters = kQChunk + 3 * kXcds * kChunkStride;
+constexpr int kQChunk = 4 * kXcds, kQCounters = kQChunk + 3 * kChunkShares * kChunkStride;
 constexpr uint32_t kChunkRays = 256;   // rays per work chunk (a multiple of 64)
 
 struct RenderArgs {

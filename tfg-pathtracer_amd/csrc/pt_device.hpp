@@ -956,13 +956,16 @@ struct BvhPtr {
 // c % parts; share x is counted by ctr[x * stride].  A wave draws from its
 // home share first, then from the others, so waves that meet cheap rays take
 // more chunks and a launch ends when the work does, not when its slowest
-// fixed range does.  Every counter starts the launch at 0.  A share is
-// skipped without an atomic once a coherent read shows it drawn out (a read
-// never runs ahead of the counter, so no chunk is lost).
+// fixed range does.  One scene uses 8 shares (one per XCD: the waves of an
+// XCD draw neighbouring chunks, which keeps their rays coherent in its L2),
+// a batch 64 (measured faster there).  Every counter starts the launch at 0.  A share is skipped without an atomic once a coherent read shows it
+// drawn out (a read never runs ahead of the counter, so no chunk is lost).
+constexpr uint32_t kWorkShares = 64;   // shares of a wide WorkSource (one per lane of a wave)
 struct WorkSource {
     uint32_t begin, end;
     uint32_t* ctr;
     uint32_t n, chunk, stride, parts, home;
+    uint32_t chunks;   // ceil(n / chunk)
 };
 
 template <int W, bool ANY, bool COUNT, bool PER_RAY_BVH = false, class Load, class Store>
@@ -987,7 +990,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
     const uint32_t lane = __lane_id();
     const unsigned long long ltMask = lane ? (~0ull >> (64 - lane)) : 0ull;
     uint32_t cursor = ws.begin, end = ws.end;   // wave-uniform
-    uint32_t tried = 0;                         // dynamic: shares found empty
+    uint32_t tried = 0;                         // narrow source: shares found drawn out
     bool more = true;                           // wave-uniform: items may remain
     bool live = false;
     uint32_t item = 0;
@@ -1001,7 +1004,34 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
         const bool want = nIdle >= refill || nIdle == 64;
         if (want && cursor >= end) {
             more = false;
-            while (ws.ctr && tried < ws.parts) {
+            // Wide sources (batches, parts == 64): lane l looks at share
+            // (home + l) % 64 at once (one load per lane instead of 64
+            // dependent loads; the whole wave runs here), and the wave draws
+            // from the first share with chunks left.  Share x holds the
+            // chunks x, x + 64, ...: (chunks - x + 63) / 64 of them.
+            while (PER_RAY_BVH && ws.ctr) {
+                const uint32_t x = (ws.home + lane) & (kWorkShares - 1);
+                const uint32_t seen =
+                    __hip_atomic_load(ws.ctr + x * ws.stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const bool left = x < ws.chunks && seen < ((ws.chunks - x + kWorkShares - 1) >> 6);
+                const unsigned long long have = __ballot(left);
+                if (!have) break;
+                const uint32_t pick = (ws.home + (uint32_t)(__ffsll((long long)have) - 1)) & (kWorkShares - 1);
+                uint32_t k = 0;
+                if (lane == 0) k = atomicAdd(ws.ctr + pick * ws.stride, 1u);
+                k = __builtin_amdgcn_readfirstlane(k);
+                const uint64_t first = ((uint64_t)pick + (uint64_t)kWorkShares * k) * ws.chunk;
+                if (first < ws.n) {
+                    cursor = (uint32_t)first;
+                    end = (uint32_t)(first + ws.chunk < ws.n ? first + ws.chunk : ws.n);
+                    more = true;
+                    break;
+                }
+            }
+            // Narrow sources (one scene, parts == 8): one share at a time,
+            // from the wave's home share on; a share found drawn out is not
+            // looked at again.
+            while (!PER_RAY_BVH && ws.ctr && tried < ws.parts) {
                 const uint32_t x = (ws.home + tried) % ws.parts;
                 uint32_t* const c = ws.ctr + x * ws.stride;
                 const uint32_t seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
