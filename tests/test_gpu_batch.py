@@ -133,3 +133,30 @@ def test_batch_culling_is_exact(batch_scenes):
         np.testing.assert_array_equal(img, culled[i][0])
     assert r_cull <= r_all
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_with_texture_maps_and_lobes_equals_standalone(gpu_ctx, scene_root):
+    """A batch mixing the scene with every texture-map kind and Disney lobe
+    (render_scenes.maps_scene), a textured sphere field loaded from files
+    (8-bit textures) and an untextured one: each scene's image equals its
+    standalone render (counts exactly; radiance to the FP32 contraction
+    differences of the one-scene and many-scene shading code, which the
+    maps and lobes amplify on a few pixels: rtol 1e-3, mean 1e-5)."""
+    from render_scenes import maps_scene
+    print("[batch] textures, maps and lobes", flush=True)
+    rng = np.random.default_rng(9)
+    tex = (rng.uniform(0, 1, size=(32, 32, 3)) * 255).astype(np.uint8)
+    d = scenegen.sphere_field(scene_root / "batch_tex", n_spheres=10, res=(40, 32), seed=4, ground_texture=tex)
+    scenes = [(maps_scene(res=(48, 36)), 3, 21), (eleven.SceneData.load(d), 4, 22),
+              (scenegen.sphere_field_arrays(n_spheres=5, res=(24, 24), seed=23), 2, 23)]
+    b = _batch(scenes)
+    b.render(depth=DEPTH)
+    for i, (s, spp, seed) in enumerate(scenes):
+        img, cnt = b.read(i)
+        ref, rcnt = _standalone(gpu_ctx, s, spp, seed, i)
+        np.testing.assert_array_equal(cnt, rcnt)
+        np.testing.assert_allclose(img, ref, rtol=1e-3, atol=1e-6)
+        assert np.mean(np.abs(img - ref) / (np.abs(ref) + 1e-3)) < 1e-5
+        assert np.isfinite(img).all() and img[..., :3].max() > 0
+    b.close()
