@@ -53,7 +53,12 @@ extern "C" {
                                    so that el_adjoint replays every path once
                                    instead of twice; needs a cleared
                                    accumulation (the record covers this one
-                                   render)                                    */
+                                   render).  Wavefront renders with max_depth
+                                   <= 16 also keep every vertex's hit and
+                                   shadow answers (16 B per vertex + 4 B per
+                                   path, when that fits a quarter of the free
+                                   device memory): el_adjoint then traces no
+                                   ray.  The image is the same either way.   */
 
 /* Passes, same numbering as the reference's `enum Passes` (kernel.h:7). */
 #define EL_PASS_BEAUTY     0
@@ -243,7 +248,8 @@ int  el_set_camera(el_ctx* ctx, const el_camera* camera);
  * nothing is copied out, and the sums stay in the context for following
  * EL_ADJ_ACCUMULATE calls.  If that render used
  * EL_FLAG_RECORD_PATHS, each path is walked once (the recorded radiance
- * gives the radiance after each vertex); otherwise twice. */
+ * gives the radiance after each vertex), from its recorded vertices when the
+ * render kept them; otherwise twice, tracing its rays. */
 int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,
                 float* d_materials, float* d_texture, int device_ptrs);
 

@@ -172,6 +172,37 @@ def test_adjoint_error_paths(gpu_ctx, small):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("flags", [0, eleven.FLAG_NO_CULL], ids=["cull", "no_cull"])
+def test_recorded_vertices_replay_without_tracing(gpu_ctx, small, flags):
+    """A wavefront render with EL_FLAG_RECORD_PATHS also keeps every vertex's
+    closest hit and both shadow answers.  Its image and sample counts equal
+    the render without the flag bit for bit (the shadow queries it adds have
+    zero contribution), and el_adjoint then replays the recorded vertices,
+    tracing nothing, to the gradients of the traced replay.  Without culling
+    the recording holds vertices past the point where the replay stops."""
+    print(f"[adjoint] recorded vertices, flags {flags}", flush=True)
+    scene, _ = small
+    H, W = scene.camera.y_res, scene.camera.x_res
+    gpu_ctx.upload(scene)
+    w = np.random.default_rng(6).normal(size=(H, W, 3)).astype(np.float32)
+    gpu_ctx.clear()
+    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, flags=flags)
+    img0, n0 = gpu_ctx.read()[:2]
+    m1, t1 = gpu_ctx.adjoint(w, grad_texture=0)
+    assert gpu_ctx.stats().adjoint_rays > 0
+    gpu_ctx.clear()
+    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, flags=flags | eleven.FLAG_RECORD_PATHS)
+    img1, n1 = gpu_ctx.read()[:2]
+    assert np.array_equal(n0, n1) and np.array_equal(img0, img1)
+    m2, t2 = gpu_ctx.adjoint(w, grad_texture=0)
+    assert gpu_ctx.stats().adjoint_rays == 0
+    assert np.abs(m1).max() > 0 and np.abs(t1).max() > 0
+    # the recorded radiance sums its terms in another order than the walk
+    assert np.linalg.norm(m2 - m1) <= 1e-4 * np.linalg.norm(m1)
+    assert np.linalg.norm(t2 - t1) <= 1e-4 * np.linalg.norm(t1)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mode", [eleven.MODE_WAVEFRONT, eleven.MODE_PLAIN])
 def test_adjoint_from_recorded_paths_equals_two_walk_adjoint(gpu_ctx, small, mode):
     """With EL_FLAG_RECORD_PATHS the adjoint takes each path's radiance from

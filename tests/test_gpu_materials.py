@@ -62,15 +62,20 @@ def test_maps_gpu_matches_oracle_common_random_numbers(gpu_ctx, maps):
 
 
 @pytest.mark.gpu
-def test_maps_adjoint_matches_oracle_adjoint(gpu_ctx, maps):
-    print("[materials] adjoint vs oracle adjoint", flush=True)
+@pytest.mark.parametrize("flags", [0, eleven.FLAG_RECORD_PATHS], ids=["traced", "recorded"])
+def test_maps_adjoint_matches_oracle_adjoint(gpu_ctx, maps, flags):
+    """The replay traces its paths again (traced) or reads the vertices the
+    recording wavefront render kept (recorded)."""
+    print(f"[materials] adjoint vs oracle adjoint, flags {flags}", flush=True)
     s, rb = maps
     H, W = s.camera.y_res, s.camera.x_res
     gpu_ctx.upload(s)
     gpu_ctx.clear()
-    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, mode=eleven.MODE_WAVEFRONT)
+    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, mode=eleven.MODE_WAVEFRONT, flags=flags)
     w = np.random.default_rng(3).normal(size=(H, W, 3)).astype(np.float32)
     dmat, dtex = gpu_ctx.adjoint(w, grad_texture=0)
+    if flags:
+        assert gpu_ctx.stats().adjoint_rays == 0
     om, ot, _ = orc.adjoint(s, rb, w, SPP, depth=DEPTH, seed=SEED, precision=0, shadow_mode=1, grad_texture=0)
     rm = np.linalg.norm(dmat - om) / np.linalg.norm(om)
     rt = np.linalg.norm(dtex - ot) / np.linalg.norm(ot)

@@ -103,6 +103,8 @@ struct el_ctx {
     uint32_t accDepth = 0, accScene = 0;
     bool accMixed = false;         // renders since el_clear used different seeds or depths
     DevBuf dRec;                   // per (sample, pixel) path radiance of a recorded render
+    DevBuf dRecHits, dRecVis, wVis;  // its vertices (wavefront recordings: RenderArgs::recHits / recVis)
+    bool recHits = false;          // with recValid: dRecHits / dRecVis hold the recorded render's vertices
     bool adjSumValid = false;      // aMat / aTex hold the sums of the el_adjoint calls so far
     int32_t adjSumTex = -1;        // their grad_texture
     bool recValid = false;         // dRec covers the whole accumulation
@@ -606,10 +608,27 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     a.sample = c->sampleBase;
     a.cull = (p->flags & EL_FLAG_NO_CULL) ? 0u : 1u;
     const bool record = (p->flags & EL_FLAG_RECORD_PATHS) != 0;
+    bool recordHits = false;
     if (record) {
         if (c->sampleBase != 0) return fail(c, EL_ERR_STATE, "EL_FLAG_RECORD_PATHS needs a cleared accumulation");
         EL_HIP(c, c->dRec.ensure((size_t)p->spp * npx * sizeof(float4)));
         a.rec = c->dRec.as<float4>();
+        // Wavefront renders also keep every vertex's hit and shadow answers
+        // (16 B per vertex + 4 B per path) when they fit a quarter of the
+        // free memory, so that el_adjoint replays without tracing.
+        if (p->mode != EL_MODE_PLAIN && depth <= (uint32_t)dev::kAdjMaxDepthHost) {
+            size_t freeB = 0, totalB = 0;
+            EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
+            const size_t hitBytes = (size_t)p->spp * npx * depth * sizeof(float4);
+            recordHits = hitBytes <= c->dRecHits.n || hitBytes <= freeB / 4;
+            if (recordHits) {
+                EL_HIP(c, c->dRecHits.ensure(hitBytes));
+                EL_HIP(c, c->dRecVis.ensure((size_t)p->spp * npx * sizeof(uint32_t)));
+                a.recHits = c->dRecHits.as<float4>();
+                a.recVis = c->dRecVis.as<uint32_t>();
+                a.recStride = (uint64_t)p->spp * npx;
+            }
+        }
     }
     c->recValid = false;
 
@@ -634,6 +653,10 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         WfState q{};
         int rc = wfPlan(c, lc, total, npad, p->queue_cap, aov, q, Q);
         if (rc) return rc;
+        if (recordHits) {
+            EL_HIP(c, c->wVis.ensure(Q * sizeof(uint32_t)));
+            q.pathVis = c->wVis.as<uint32_t>();
+        }
         a.npad = (uint32_t)npad;
         size_t ev = 2;
         std::vector<std::pair<size_t, int>> timed;   // (event index, kind) kind 0 extend 1 shade 2 shadow
@@ -661,8 +684,11 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                 ev += 2;
                 hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(z0, c->stream));
+                if (recordHits) q.visBit = 1u << (2 * b);
                 EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowH, q.counters + kQShadowH, a.genCount, ctr));
+                if (recordHits) q.visBit = 1u << (2 * b + 1);
                 EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowP, q.counters + kQShadowP, a.genCount, ctr));
+                q.visBit = 0;
                 EL_HIP(c, hipEventRecord(z1, c->stream));
                 timed.push_back({ev, 2});
                 ev += 2;
@@ -697,6 +723,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     }
     c->sampleBase += p->spp;
     c->recValid = record;
+    c->recHits = recordHits;
     c->totalRays += tally(h, kStatRays);
     c->totalVerts += tally(h, kStatVerts);
     c->totalPaths += tally(h, kStatPaths);
@@ -784,6 +811,13 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     a.maxDepth = c->accDepth;
     a.sample = 0;
     a.cull = 1;   // exact: a zero-throughput path adds neither radiance nor gradient
+    if (c->recValid && c->recHits) {
+        // the recording stopped no later than this replay does (it culled
+        // alike or not at all), so every vertex the replay reads is recorded
+        a.recHits = c->dRecHits.as<float4>();
+        a.recVis = c->dRecVis.as<uint32_t>();
+        a.recStride = (uint64_t)c->sampleBase * npx;
+    }
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
     if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
     EL_HIP(c, hipEventRecord(e0, c->stream));

@@ -361,11 +361,17 @@ struct PathVertex {
 // `sample`: camera ray, up to maxDepth vertices with DisneySample / shade and
 // both shadow queries, escape to the environment.  Returns the radiance
 // before the clamp; `hook` sees every shaded vertex; `verts` counts them.
-template <int W, class Hook>
+// REPLAY: the closest hits and shadow answers come from a recording
+// wavefront render of the same path (a.recHits / a.recVis) instead of being
+// traced; the path stops where the recording's did, so every record read was
+// written by it (a triangle index out of range reads as an escape).
+template <int W, bool REPLAY = false, class Hook>
 __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, uint32_t x, uint32_t y,
                                        uint32_t idx, uint32_t sample, TravStack& st, TraceCounters& cnt,
                                        uint32_t& rays, uint32_t& verts, Hook&& hook) {
     PathRng rng{a.k0, a.k1, idx, sample, a.scene};
+    const uint64_t recAt = (uint64_t)sample * s.cam.xRes * s.cam.yRes + idx;
+    const uint32_t vis = REPLAY ? a.recVis[recAt] : 0u;
     uint4 c = rng.block(0);
     uint32_t c5 = rng.block(1).x;
     v3 o, d;
@@ -373,8 +379,18 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
     v3 L = mk(0, 0, 0), T = mk(1, 1, 1);
     int i;
     for (i = 0; i < (int)a.maxDepth; i++) {
-        rays++;
-        TraceResult h = traceRay<W, false, false>(s, o.x, o.y, o.z, d.x, d.y, d.z, __int_as_float(0x7f800000), st, cnt);
+        TraceResult h;
+        if constexpr (REPLAY) {
+            const float4 r = a.recHits[(uint64_t)i * a.recStride + recAt];
+            h.tri = __float_as_int(r.x);
+            h.t = r.y;
+            h.u = r.z;
+            h.v = r.w;
+            if (h.tri >= s.nTris) h.tri = -1;
+        } else {
+            rays++;
+            h = traceRay<W, false, false>(s, o.x, o.y, o.z, d.x, d.y, d.z, __int_as_float(0x7f800000), st, cnt);
+        }
         if (h.tri < 0) {
             L = L + envLookup(s, d) * T;
             break;
@@ -384,18 +400,25 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
         uint4 rb = rng.block(2 + i);
         uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
         VertexOut vo = shadeVertex(s, d, T, sh, hd, r);
-        rays++;
-        TraceResult hs = traceRay<W, true, false>(s, vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, vo.hDir.x, vo.hDir.y,
-                                                  vo.hDir.z, __int_as_float(0x7f800000), st, cnt);
+        bool visH;
+        if constexpr (REPLAY) {
+            visH = (vis >> (2 * i)) & 1u;
+        } else {
+            rays++;
+            visH = traceRay<W, true, false>(s, vo.hOrig.x, vo.hOrig.y, vo.hOrig.z, vo.hDir.x, vo.hDir.y, vo.hDir.z,
+                                            __int_as_float(0x7f800000), st, cnt).tri < 0;
+        }
         v3 add = vo.base;
-        const bool visH = hs.tri < 0;
         if (visH) add = add + vo.contribH;
         bool visP = false;
         if (vo.hasP) {
-            rays++;
-            TraceResult hp = traceRay<W, true, false>(s, vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, vo.pDir.x, vo.pDir.y,
-                                                      vo.pDir.z, vo.pTmax, st, cnt);
-            visP = hp.tri < 0;
+            if constexpr (REPLAY) {
+                visP = (vis >> (2 * i + 1)) & 1u;
+            } else {
+                rays++;
+                visP = traceRay<W, true, false>(s, vo.pOrig.x, vo.pOrig.y, vo.pOrig.z, vo.pDir.x, vo.pDir.y,
+                                                vo.pDir.z, vo.pTmax, st, cnt).tri < 0;
+            }
             if (visP) add = add + vo.contribP;
         }
         L = L + add;
@@ -570,12 +593,15 @@ __device__ __forceinline__ void waveAddGradients(int m, const float g[6], double
 // gradients of the selected texture go straight to FP32 atomics.
 constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
 
-template <int W>
+// REPLAY: the paths' vertices come from the recording render (walkPath), so
+// the replay traces nothing.
+template <int W, bool REPLAY>
 EL_KERNEL(EL_WPE_ADJ) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
                                        const uint32_t* __restrict__ counts, const float4* __restrict__ rec,
                                        int gradTex, double* __restrict__ dmat, float* __restrict__ dtex, int* gstack,
                                        uint32_t gthreads, unsigned long long* counters) {
-    TravStack st = makeStack(s, gstack, gthreads);
+    TravStack st{};
+    if constexpr (!REPLAY) st = makeStack(s, gstack, gthreads);
     TraceCounters cnt{0, 0};
     const uint32_t W_ = s.cam.xRes, H_ = s.cam.yRes;
     const uint32_t tilesX = (W_ + 15) / 16;
@@ -601,17 +627,17 @@ EL_KERNEL(EL_WPE_ADJ) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, con
                 const uint32_t n = counts[idx];
                 uint32_t vdummy = 0;
                 v3 Lt;
-                if (rec) {
+                if (REPLAY || rec) {
                     const float4 r = rec[(uint64_t)sample * W_ * H_ + idx];
                     Lt = mk(r.x, r.y, r.z);
-                } else {
+                } else if constexpr (!REPLAY) {
                     Lt = walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, vdummy, [](const PathVertex&) {});
                 }
                 paths++;
                 float ad[3];
                 adjointWeight(Lt, dimage, idx, n, ad);
                 if (ad[0] != 0.f || ad[1] != 0.f || ad[2] != 0.f) {
-                    walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
+                    walkPath<W, REPLAY>(s, a, x, y, idx, sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
                         float g[6];
                         const int mat = vertexGradient(s, pv.d, pv.T, *pv.sh, *pv.hd, *pv.vo, pv.visH, pv.visP,
                                                        Lt - pv.Lincl, ad, gradTex, dtex, g);
@@ -681,6 +707,7 @@ EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
             if (a.cull) quiet = nanProneBounces(s, rng, (int)a.maxDepth);
         }
         q.pathQuiet[slot] = quiet;
+        if (q.pathVis) q.pathVis[slot] = 0u;
         float4* Q = q.ray[0] + 2 * (uint64_t)slot;
         Q[0] = make_float4(o.x, o.y, o.z, __uint_as_float(slot));
         Q[1] = make_float4(d.x, d.y, d.z, __uint_as_float(a.scene));
@@ -814,6 +841,18 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                     addRadiance(q.pathL, slot, bounce, ev);
                 }
                 hit = tri >= 0;
+                if constexpr (!P::kMany) {
+                    // recording render: vertex `bounce` of this path, hit or
+                    // escape (tri -2: camera-queue padding, no path)
+                    if (a.recHits && tri >= -1) {
+                        const DevScene& s = sc.get(0);
+                        uint32_t smp, px, py;
+                        const uint32_t pix = slotPixel(s, a, __float_as_uint(rayIn[2 * (uint64_t)e].w), smp, px, py);
+                        if (pix != 0xFFFFFFFFu)
+                            a.recHits[(uint64_t)bounce * a.recStride + (uint64_t)smp * s.cam.xRes * s.cam.yRes +
+                                      pix] = q.hits[e];
+                    }
+                }
             }
             const unsigned long long m = __ballot(hit);
             if (lane == 0) s_wcnt[wave] = (uint32_t)__popcll(m);
@@ -876,9 +915,12 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             }
             verts++;
             rays += 1 + (vo.hasP ? 1 : 0);
-            // A shadow ray whose contribution is exactly zero cannot change L.
-            wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
-            wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
+            // A shadow ray whose contribution is exactly zero cannot change L;
+            // a recording render traces every query (the adjoint's
+            // derivative terms need the answer where the value is zero).
+            const bool recV = !P::kMany && q.pathVis != nullptr;
+            wantH = recV || vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
+            wantP = vo.hasP && (recV || vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
             cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
             // only an extended path reads its throughput again
             if (cont) q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
@@ -946,7 +988,9 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
         if (h.tri >= 0) return;
         const uint64_t e = qm.entry(i, segSize);
         const uint32_t slot = __float_as_uint(sh[3 * e].w);
+        if (q.visBit) q.pathVis[slot] |= q.visBit;
         const float4 C = sh[3 * e + 2];
+        if (C.x == 0.f && C.y == 0.f && C.z == 0.f) return;   // traced only for the record
         const float4 L = q.pathL[slot];
         q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
     };
@@ -979,6 +1023,7 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
             const float4 Lf = q.pathL[slot];
             paths++;
             if (a.rec) a.rec[(uint64_t)sample * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
+            if (a.recVis) a.recVis[(uint64_t)sample * s.cam.xRes * s.cam.yRes + idx] = q.pathVis[slot];
             const v3 L = clamp3(mk(Lf.x, Lf.y, Lf.z));
             if (anyNan(L)) continue;
             acc = make_float4(acc.x + L.x, acc.y + L.y, acc.z + L.z, acc.w);
@@ -1180,12 +1225,15 @@ hipError_t launchAdjoint(const LaunchCfg& c, const DevScene& s, const RenderArgs
     uint64_t total = (uint64_t)((s.cam.xRes + 15) / 16) * ((s.cam.yRes + 15) / 16) * 256 * spp;
     uint32_t grid = gridFor(total, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
-    if (s.width == 4)
-        hipLaunchKernelGGL(k_adjoint<4>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, rec, gradTex,
-                           dmat, dtex, c.gstack, c.maxThreads, counters);
+    if (a.recHits)   // replays recorded vertices: no traversal, no stack
+        hipLaunchKernelGGL((k_adjoint<4, true>), dim3(grid), dim3(c.block), 0, c.stream, s, a, spp, dimage, counts,
+                           rec, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
+    else if (s.width == 4)
+        hipLaunchKernelGGL((k_adjoint<4, false>), dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts,
+                           rec, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
     else
-        hipLaunchKernelGGL(k_adjoint<8>, dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts, rec, gradTex,
-                           dmat, dtex, c.gstack, c.maxThreads, counters);
+        hipLaunchKernelGGL((k_adjoint<8, false>), dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts,
+                           rec, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
     return hipGetLastError();
 }
 

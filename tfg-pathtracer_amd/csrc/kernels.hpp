@@ -52,6 +52,14 @@ struct RenderArgs {
     uint32_t npad;          // wavefront: padded pixel count (8x8 tiles)
     uint32_t cull;          // stop zero-throughput paths that cannot turn NaN
     float4* rec;            // per (sample, pixel): the path's radiance before the clamp, or null
+    // Wavefront renders that record, and adjoints that replay, path vertices
+    // (one scene; all null otherwise): the closest hit (tri, t, u, v; tri -1:
+    // escaped) of vertex b of path (sample, pixel) at recHits[b * recStride +
+    // sample * pixels + pixel], and per path the shadow queries that found
+    // the light visible: bit 2b HDRI, bit 2b + 1 point light of vertex b.
+    float4* recHits;
+    uint32_t* recVis;
+    uint64_t recStride;
 };
 
 struct WfState {
@@ -65,6 +73,8 @@ struct WfState {
     uint32_t* pathSample;   // batches: sample index per path slot (one scene: slotPixel)
     uint32_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b)
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
+    uint32_t* pathVis;      // recording renders: visibility bits per path slot (RenderArgs::recVis), else null
+    uint32_t visBit;        // shadow launches of a recording render: the bit a visible query sets
     uint32_t* counters;     // kQCounters: per-XCD segment counts of the four queues
     uint32_t segSize;       // entries per XCD segment of a compacted queue
     int* gstack;
