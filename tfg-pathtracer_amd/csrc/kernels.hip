@@ -954,7 +954,9 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
 // Any-hit test of a shadow queue; an unoccluded entry adds its contribution
 // to its path's radiance (each path has at most one entry per queue, and the
 // two shadow queues are separate launches, so the update needs no atomics).
-template <int W, class P>
+// REC: a recording render's launch; visible queries set q.visBit in the
+// path's visibility word, and queries traced only for the record add nothing.
+template <int W, class P, bool REC = false>
 EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restrict__ sh,
                                         const uint32_t* countIn, uint32_t refill,
                                         unsigned long long* counters) {
@@ -988,9 +990,11 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
         if (h.tri >= 0) return;
         const uint64_t e = qm.entry(i, segSize);
         const uint32_t slot = __float_as_uint(sh[3 * e].w);
-        if (q.visBit) q.pathVis[slot] |= q.visBit;
         const float4 C = sh[3 * e + 2];
-        if (C.x == 0.f && C.y == 0.f && C.z == 0.f) return;   // traced only for the record
+        if constexpr (REC) {
+            q.pathVis[slot] |= q.visBit;
+            if (C.x == 0.f && C.y == 0.f && C.z == 0.f) return;   // traced only for the record
+        }
         const float4 L = q.pathL[slot];
         q.pathL[slot] = make_float4(L.x + C.x, L.y + C.y, L.z + C.z, L.w);
     };
@@ -1292,9 +1296,16 @@ hipError_t launchWfShadow(const LaunchCfg& c, const DevScene& s, const WfState& 
                           const uint32_t* countIn, uint32_t maxCount, unsigned long long* counters) {
     uint32_t grid = gridFor(maxCount, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
-    if (s.width == 4)
+    const bool rec = q.visBit != 0;
+    if (s.width == 4 && rec)
+        hipLaunchKernelGGL((k_wf_shadow<4, OneScene, true>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s},
+                           q, shadow, countIn, c.refill, counters);
+    else if (s.width == 4)
         hipLaunchKernelGGL((k_wf_shadow<4, OneScene>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s}, q,
                            shadow, countIn, c.refill, counters);
+    else if (rec)
+        hipLaunchKernelGGL((k_wf_shadow<8, OneScene, true>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s},
+                           q, shadow, countIn, c.refill, counters);
     else
         hipLaunchKernelGGL((k_wf_shadow<8, OneScene>), dim3(grid), dim3(c.block), lds, c.stream, OneScene{s}, q,
                            shadow, countIn, c.refill, counters);
