@@ -915,12 +915,19 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             }
             verts++;
             rays += 1 + (vo.hasP ? 1 : 0);
-            // A shadow ray whose contribution is exactly zero cannot change L;
-            // a recording render traces every query (the adjoint's
-            // derivative terms need the answer where the value is zero).
+            // A shadow ray whose contribution is exactly zero cannot change L.
+            // A recording render also traces such a query when the direction
+            // lies in the BRDF's support, where the adjoint's derivative
+            // term may be nonzero although the value is zero (a black
+            // albedo); outside it disneyEvalGrad is zero and the answer is
+            // not needed (the 1e-6 margins absorb rounding differences
+            // between this test and the one the adjoint makes).
             const bool recV = !P::kMany && q.pathVis != nullptr;
-            wantH = recV || vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f;
-            wantP = vo.hasP && (recV || vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f);
+            const bool front = recV && hd.transmission < 1.0f && dot(hd.normal, d) < 1e-6f;
+            wantH = vo.contribH.x != 0.f || vo.contribH.y != 0.f || vo.contribH.z != 0.f ||
+                    (front && dot(hd.normal, vo.hdir) > -1e-6f);
+            wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f ||
+                                (front && dot(hd.normal, vo.ldir) > -1e-6f));
             cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
             // only an extended path reads its throughput again
             if (cont) q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
