@@ -44,6 +44,9 @@ constexpr int kMaxWaves = kMaxBlock / 64;
 #ifndef EL_WPE_ADJ
 #define EL_WPE_ADJ 4
 #endif
+#ifndef EL_WPE_ADJ_REPLAY   // the adjoint over recorded vertices (no traversal)
+#define EL_WPE_ADJ_REPLAY 3
+#endif
 #define EL_KERNEL(wpe) __global__ __launch_bounds__(kMaxBlock) __attribute__((amdgpu_waves_per_eu(wpe)))
 
 __device__ __forceinline__ uint32_t waveSumU32(uint32_t v) {
@@ -596,7 +599,7 @@ constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
 // REPLAY: the paths' vertices come from the recording render (walkPath), so
 // the replay traces nothing.
 template <int W, bool REPLAY>
-EL_KERNEL(EL_WPE_ADJ) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
+EL_KERNEL(REPLAY ? EL_WPE_ADJ_REPLAY : EL_WPE_ADJ) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
                                        const uint32_t* __restrict__ counts, const float4* __restrict__ rec,
                                        int gradTex, double* __restrict__ dmat, float* __restrict__ dtex, int* gstack,
                                        uint32_t gthreads, unsigned long long* counters) {
