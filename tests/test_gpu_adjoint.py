@@ -172,15 +172,17 @@ def test_adjoint_error_paths(gpu_ctx, small):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("flags", [0, eleven.FLAG_NO_CULL], ids=["cull", "no_cull"])
-def test_recorded_vertices_replay_without_tracing(gpu_ctx, small, flags):
+@pytest.mark.parametrize("flags,queue_cap", [(0, 0), (eleven.FLAG_NO_CULL, 0), (0, 1024)],
+                         ids=["cull", "no_cull", "generations"])
+def test_recorded_vertices_replay_without_tracing(gpu_ctx, small, flags, queue_cap):
     """A wavefront render with EL_FLAG_RECORD_PATHS also keeps every vertex's
     closest hit and both shadow answers.  Its image and sample counts equal
     the render without the flag bit for bit (the shadow queries it adds have
     zero contribution), and el_adjoint then replays the recorded vertices,
     tracing nothing, to the gradients of the traced replay.  Without culling
-    the recording holds vertices past the point where the replay stops."""
-    print(f"[adjoint] recorded vertices, flags {flags}", flush=True)
+    the recording holds vertices past the point where the replay stops; with
+    a small queue the recording spans several generations."""
+    print(f"[adjoint] recorded vertices, flags {flags}, queue {queue_cap}", flush=True)
     scene, _ = small
     H, W = scene.camera.y_res, scene.camera.x_res
     gpu_ctx.upload(scene)
@@ -191,7 +193,7 @@ def test_recorded_vertices_replay_without_tracing(gpu_ctx, small, flags):
     m1, t1 = gpu_ctx.adjoint(w, grad_texture=0)
     assert gpu_ctx.stats().adjoint_rays > 0
     gpu_ctx.clear()
-    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, flags=flags | eleven.FLAG_RECORD_PATHS)
+    gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, flags=flags | eleven.FLAG_RECORD_PATHS, queue_cap=queue_cap)
     img1, n1 = gpu_ctx.read()[:2]
     assert np.array_equal(n0, n1) and np.array_equal(img0, img1)
     m2, t2 = gpu_ctx.adjoint(w, grad_texture=0)
