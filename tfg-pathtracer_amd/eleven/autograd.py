@@ -62,7 +62,8 @@ class Renderer:
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
         self.ctx.clear()
-        # the recorded path radiance lets the backward replay each path once
+        # the recorded paths (radiance; wavefront: also every vertex) let the
+        # backward replay each path once, without tracing in wavefront mode
         self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode, flags=FLAG_RECORD_PATHS)
         return self.ctx.read()[0][..., :3].copy() if read else None
 

@@ -68,8 +68,8 @@ class MultiView:
 
     def render_view(self, v: int, seed: int, spp: int | None = None, record: bool = False, read: bool = True):
         """Mean image of view v (None with read=False); with `record`, the
-        render keeps its path radiance for a following el_adjoint (which then
-        walks each path once)."""
+        render keeps its paths for a following el_adjoint (which then walks
+        each path once, from the recorded vertices in wavefront mode)."""
         self.ctx.set_camera(self.cameras[v])
         self.ctx.render(spp or self.spp, depth=self.depth, seed=seed, mode=self.mode,
                         flags=FLAG_RECORD_PATHS if record else 0)
