@@ -86,13 +86,31 @@ def copy_bandwidth(gib=2, reps=5):
     return best
 
 
+def cpu_threads():
+    """Host threads for the CPU baselines: every core this process may run on."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, budget_s=15.0, threads=None):
     """Oracle render (FP32, reference estimator) of a centred crop of the frame,
     its sample count chosen from a short probe so the timed part takes about
     `budget_s` seconds on `threads` host cores."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle_py as orc
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or cpu_threads()
     cam = scene.camera
     t0 = time.perf_counter()
     ref = orc.RefBVH(scene.vertices())
@@ -120,7 +138,10 @@ def adjoint_bench(steps=5, with_cpu=True):
     gradient covers 65 materials x 6 parameters + a 256x256 RGB albedo texture
     (196,608 texels).  One gradient = forward render (wavefront) + adjoint
     (path replay) of an L2 loss against a target rendered with perturbed
-    parameters; parameter and texture uploads included."""
+    parameters; parameter and texture uploads included.  The headline is the
+    O(pixels) replay (nothing of a path kept between forward and adjoint);
+    `recorded` is the same step with EL_FLAG_RECORD_PATHS (every vertex of
+    every path kept by the forward render), reported beside it."""
     import eleven
     from eleven import scenegen
     from eleven.autograd import Renderer, material_params
@@ -129,37 +150,50 @@ def adjoint_bench(steps=5, with_cpu=True):
     scene = eleven.SceneData.load(d)
     ctx = eleven.Context(0)
     ctx.upload(scene)
-    r = Renderer(ctx, scene, spp=32, depth=4, seed=1, grad_texture=0)
     p0 = material_params(scene)
     t0 = scene.texture(0).copy()
     rng = np.random.default_rng(0)
     pt = np.clip(p0 + rng.normal(0, 0.1, p0.shape).astype(np.float32), 0.0, 1.0)
     pt[:, 3] = np.maximum(pt[:, 3], 0.05)
-    target = r.render(pt, t0)
-    img = r.render(p0, t0)
-    ctx.adjoint(2 * (img - target), grad_texture=0)      # warm-up
-    fwd_ms = adj_ms = 0.0
-    t_start = time.perf_counter()
-    for _ in range(steps):
-        # forward with the new parameters and texture, the L2 residual formed
-        # on the device (el_residual_l2), then the adjoint of that residual
-        r.render(p0, t0, read=False)
-        fwd_ms += ctx.stats().render_ms
-        ctx.residual_l2(target)
-        dmat, dtex = ctx.adjoint(None, grad_texture=0)
-        adj_ms += ctx.stats().adjoint_ms
-    wall = time.perf_counter() - t_start
+    legs = {}
+    for record in (False, True):
+        r = Renderer(ctx, scene, spp=32, depth=4, seed=1, grad_texture=0, record=record)
+        target = r.render(pt, t0)
+        img = r.render(p0, t0)
+        ctx.adjoint(2 * (img - target), grad_texture=0)      # warm-up
+        fwd_ms = adj_ms = 0.0
+        t_start = time.perf_counter()
+        for _ in range(steps):
+            # forward with the new parameters and texture, the L2 residual formed
+            # on the device (el_residual_l2), then the adjoint of that residual
+            r.render(p0, t0, read=False)
+            st = ctx.stats()
+            fwd_ms += st.render_ms
+            ctx.residual_l2(target)
+            dmat, dtex = ctx.adjoint(None, grad_texture=0)
+            adj_ms += ctx.stats().adjoint_ms
+        wall = time.perf_counter() - t_start
+        sa = ctx.stats()
+        legs[record] = {"value": round(steps / wall, 3), "unit": "grads/s", "ms_per_grad": round(wall / steps * 1e3, 2),
+                        "forward_ms": round(fwd_ms / steps, 3), "adjoint_ms": round(adj_ms / steps, 3),
+                        "adjoint_rays": int(sa.adjoint_rays),
+                        "adjoint_mem_mb": round(sa.adjoint_bytes / 2 ** 20, 1),
+                        "record_mem_mb": round(st.record_bytes / 2 ** 20, 1),
+                        "grad_norm": float(np.linalg.norm(dmat))}
     img = ctx.read()[0][..., :3]
-    out = {"metric": "adjoint grads/s", "value": round(steps / wall, 3), "unit": "grads/s",
+    out = {"metric": "adjoint grads/s", "value": legs[False]["value"], "unit": "grads/s",
            "config": "sphere field 61,442 tris, 128x128, 32 spp, depth 4, gradient of 65 materials x 6 "
                      "parameters + 256x256 RGB albedo texture (L2 loss)",
-           "ms_per_grad": round(wall / steps * 1e3, 2), "forward_ms": round(fwd_ms / steps, 2),
-           "adjoint_ms": round(adj_ms / steps, 2), "grad_norm": float(np.linalg.norm(dmat))}
+           "method": "path replay, O(pixels): the adjoint re-walks every path from its RNG counters (traced "
+                     "once, replayed from the thread's own vertex slots); nothing of a path kept after the forward",
+           **{k: v for k, v in legs[False].items() if k not in ("value", "unit", "record_mem_mb")},
+           "recorded": {**legs[True], "method": "EL_FLAG_RECORD_PATHS: the forward keeps every path's radiance "
+                                                "and vertices (O(pixels x spp x depth)); the adjoint traces nothing"}}
     ctx.close()
     if with_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_py as orc
-        threads = min(16, os.cpu_count() or 1)
+        threads = cpu_threads()
         rb = orc.RefBVH(scene.vertices())
         w = (2 * (img - target)).astype(np.float32)
         t0c = time.perf_counter()

@@ -49,11 +49,14 @@ extern "C" {
                                  and sample counts are the same either way) */
 #define EL_FLAG_COUNT_TRAVERSAL 2u /* wavefront: tally node fetches and
                                       triangle tests of the closest-hit stage */
-#define EL_FLAG_RECORD_PATHS 8u /* keep each path's radiance (16 B per path)
-                                   so that el_adjoint replays every path once
-                                   instead of twice; needs a cleared
+#define EL_FLAG_RECORD_PATHS 8u /* opt-in, O(pixels x spp x depth) memory:
+                                   keep each path's radiance (16 B per path,
+                                   when that fits half of the free device
+                                   memory) so that el_adjoint walks every
+                                   path once instead of twice; needs a cleared
                                    accumulation (the record covers this one
-                                   render).  Wavefront renders with max_depth
+                                   render; el_clear and el_set_camera release
+                                   it).  Wavefront renders with max_depth
                                    <= 16 also keep every vertex's hit and
                                    shadow answers (16 B per vertex + 4 B per
                                    path, when that fits a quarter of the free
@@ -161,6 +164,11 @@ typedef struct {
                                   walk + replay)                             */
     uint64_t texels_8bit;      /* texels read from the 8-bit copies (textures
                                   from 8-bit files, not since set_texture)   */
+    uint64_t adjoint_bytes;    /* device memory the last el_adjoint used beyond
+                                  the scene and the accumulation: per-thread
+                                  vertex cache, adjoint image, gradients     */
+    uint64_t record_bytes;     /* device memory held by the last recording
+                                  render (EL_FLAG_RECORD_PATHS), 0 if none   */
 } el_stats;
 
 /* Adjoint of the accumulated image (no counterpart in the reference, which
@@ -242,14 +250,19 @@ int  el_set_camera(el_ctx* ctx, const el_camera* camera);
  * the texels of texture params->grad_texture, written to d_texture[h][w][3]
  * (may be NULL if grad_texture < 0).  d_image is [H][W][3] in the layout of
  * el_read_passes.  Each path is replayed from its RNG counters, nothing of a
- * path is stored; a material whose albedo comes from a texture gets no
- * albedo gradient (its texels do, if selected).  With `device_ptrs` != 0 the
- * three arrays are device pointers.  d_materials and d_texture may be NULL:
+ * path outlives the thread that replays it; a material whose albedo comes
+ * from a texture gets no albedo gradient (its texels do, if selected).
+ * Without a recording render
+ * each path is walked twice: once traced, for its radiance, keeping the
+ * closest hits of its vertices in the walking thread's own slots (a buffer of
+ * resident threads x depth x 16 B, independent of the pixel and sample
+ * counts), then replayed from those without tracing.  With `device_ptrs` != 0
+ * the three arrays are device pointers.  d_materials and d_texture may be NULL:
  * nothing is copied out, and the sums stay in the context for following
  * EL_ADJ_ACCUMULATE calls.  If that render used
  * EL_FLAG_RECORD_PATHS, each path is walked once (the recorded radiance
  * gives the radiance after each vertex), from its recorded vertices when the
- * render kept them; otherwise twice, tracing its rays. */
+ * render kept them. */
 int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,
                 float* d_materials, float* d_texture, int device_ptrs);
 

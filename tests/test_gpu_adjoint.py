@@ -206,10 +206,14 @@ def test_recorded_vertices_replay_without_tracing(gpu_ctx, small, flags, queue_c
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mode", [eleven.MODE_WAVEFRONT, eleven.MODE_PLAIN])
-def test_adjoint_from_recorded_paths_equals_two_walk_adjoint(gpu_ctx, small, mode):
+def test_adjoint_from_recorded_paths_equals_replay_adjoint(gpu_ctx, small, mode):
     """With EL_FLAG_RECORD_PATHS the adjoint takes each path's radiance from
     the forward render instead of walking the path a first time: the same
-    gradients up to the FP32 summation order of the two radiance sums."""
+    gradients up to the FP32 summation order of the two radiance sums.
+    Without it each path is traced once (the primal walk, which keeps its
+    vertices per thread) and replayed from those vertices; with it the plain
+    renderer's record traces only the paths whose adjoint weight is nonzero,
+    the wavefront one's nothing."""
     print(f"[adjoint] recorded vs two-walk, mode {mode}", flush=True)
     scene, _ = small
     H, W = scene.camera.y_res, scene.camera.x_res
@@ -223,7 +227,7 @@ def test_adjoint_from_recorded_paths_equals_two_walk_adjoint(gpu_ctx, small, mod
     gpu_ctx.render(SPP, depth=DEPTH, seed=SEED, mode=mode, flags=eleven.FLAG_RECORD_PATHS)
     m2, t2 = gpu_ctx.adjoint(w, grad_texture=0)
     r2 = gpu_ctx.stats().adjoint_rays
-    assert r2 < r1                       # one walk per path instead of two
+    assert r1 > 0 and r2 <= r1           # at most one traced walk per path
     assert np.linalg.norm(m2 - m1) <= 1e-4 * np.linalg.norm(m1)
     assert np.linalg.norm(t2 - t1) <= 1e-4 * np.linalg.norm(t1)
     gpu_ctx.render(1, depth=DEPTH, seed=SEED, mode=mode)   # the record no longer covers the accumulation

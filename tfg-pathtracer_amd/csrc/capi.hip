@@ -110,6 +110,7 @@ struct el_ctx {
     bool recValid = false;         // dRec covers the whole accumulation
     uint64_t totalRays = 0, totalVerts = 0, totalPaths = 0;
     DevBuf aImage, aMat, aTex;     // adjoint image, FP64 material and FP32 texel gradients
+    DevBuf aCache;                 // el_adjoint: each grid thread's vertices of the path it replays
     DevBuf aTarget, aLoss;         // el_residual_l2: target image, FP64 loss
     size_t residualPx = 0;         // aImage holds el_residual_l2's residual for this many pixels (0: none)
     el_stats stats{};
@@ -291,6 +292,17 @@ static int uploadBvh(el_ctx* c, int width) {
     return EL_OK;
 }
 
+// Releases the buffers of a recording render (EL_FLAG_RECORD_PATHS), which
+// cover only the accumulation they were recorded into.
+static void releaseRecord(el_ctx* c) {
+    c->dRec.release();
+    c->dRecHits.release();
+    c->dRecVis.release();
+    c->wVis.release();
+    c->recValid = false;
+    c->recHits = false;
+}
+
 static int clearAccum(el_ctx* c) {
     size_t npx = (size_t)c->cam.x_res * c->cam.y_res;
     EL_HIP(c, hipMemsetAsync(c->dAccum.p, 0, npx * sizeof(float4), c->stream));
@@ -298,7 +310,7 @@ static int clearAccum(el_ctx* c) {
     if (c->dAov.p) EL_HIP(c, hipMemsetAsync(c->dAov.p, 0, c->dAov.n, c->stream));
     c->sampleBase = 0;
     c->accMixed = false;
-    c->recValid = false;
+    releaseRecord(c);
     c->totalRays = c->totalVerts = c->totalPaths = 0;
     EL_HIP(c, hipStreamSynchronize(c->stream));
     return EL_OK;
@@ -481,6 +493,7 @@ int el_set_camera(el_ctx* c, const el_camera* cam) {
     if (cam->x_res != c->cam.x_res || cam->y_res != c->cam.y_res) c->dAov.release();
     c->cam = *cam;
     c->ds.cam = devCamera(*cam);
+    c->residualPx = 0;   // the residual of el_residual_l2 belongs to the previous view
     return clearAccum(c);
 }
 
@@ -515,7 +528,7 @@ static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
     EL_HIP(c, c->wL.ensure(cap * 16));
     EL_HIP(c, c->wPix.ensure(cap * 4));
     EL_HIP(c, c->wSample.ensure(cap * 4));
-    EL_HIP(c, c->wQuiet.ensure(cap * 4));
+    EL_HIP(c, c->wQuiet.ensure(cap * 8));
     if (aov) EL_HIP(c, c->wAov.ensure(cap * 48));
     EL_HIP(c, c->wCounters.ensure(kQCounters * sizeof(uint32_t)));
     c->wfCap = std::max(c->wfCap, cap);
@@ -562,7 +575,7 @@ static int wfPlan(el_ctx* c, const LaunchCfg& lc, uint64_t total, uint64_t npad,
     q.pathL = c->wL.as<float4>();
     q.pathPix = c->wPix.as<uint32_t>();
     q.pathSample = c->wSample.as<uint32_t>();
-    q.pathQuiet = c->wQuiet.as<uint32_t>();
+    q.pathQuiet = c->wQuiet.as<uint64_t>();
     q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
     q.counters = c->wCounters.as<uint32_t>();
     q.segSize = (uint32_t)segSize;
@@ -607,10 +620,18 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     a.maxDepth = depth;
     a.sample = c->sampleBase;
     a.cull = (p->flags & EL_FLAG_NO_CULL) ? 0u : 1u;
-    const bool record = (p->flags & EL_FLAG_RECORD_PATHS) != 0;
+    bool record = (p->flags & EL_FLAG_RECORD_PATHS) != 0;
     bool recordHits = false;
     if (record) {
         if (c->sampleBase != 0) return fail(c, EL_ERR_STATE, "EL_FLAG_RECORD_PATHS needs a cleared accumulation");
+        // the radiance record (16 B per path) is kept only when it fits half
+        // of the free device memory; otherwise el_adjoint replays without it
+        size_t freeB = 0, totalB = 0;
+        EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
+        const size_t recBytes = (size_t)p->spp * npx * sizeof(float4);
+        record = recBytes <= c->dRec.n || recBytes <= freeB / 2;
+    }
+    if (record) {
         EL_HIP(c, c->dRec.ensure((size_t)p->spp * npx * sizeof(float4)));
         a.rec = c->dRec.as<float4>();
         // Wavefront renders also keep every vertex's hit and shadow answers
@@ -724,6 +745,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     c->sampleBase += p->spp;
     c->recValid = record;
     c->recHits = recordHits;
+    c->stats.record_bytes = record ? c->dRec.n + c->dRecHits.n + c->dRecVis.n + c->wVis.n : 0;
     c->totalRays += tally(h, kStatRays);
     c->totalVerts += tally(h, kStatVerts);
     c->totalPaths += tally(h, kStatPaths);
@@ -818,11 +840,16 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
         a.recVis = c->dRecVis.as<uint32_t>();
         a.recStride = (uint64_t)c->sampleBase * npx;
     }
+    float4* cache = nullptr;
+    if (!c->recValid) {   // the primal walk keeps its vertices for the replay, per grid thread
+        EL_HIP(c, c->aCache.ensure((size_t)lc.maxThreads * c->accDepth * sizeof(float4)));
+        cache = c->aCache.as<float4>();
+    }
     hipEvent_t e0 = poolEvent(c, 0), e1 = poolEvent(c, 1);
     if (!e0 || !e1) return fail(c, EL_ERR_HIP, "event creation failed");
     EL_HIP(c, hipEventRecord(e0, c->stream));
     EL_HIP(c, launchAdjoint(lc, c->ds, a, c->sampleBase, c->aImage.as<float>(), c->dCounts.as<uint32_t>(),
-                            c->recValid ? c->dRec.as<float4>() : nullptr, gtex, c->aMat.as<double>(),
+                            c->recValid ? c->dRec.as<float4>() : nullptr, cache, gtex, c->aMat.as<double>(),
                             ntex ? c->aTex.as<float>() : nullptr, ctr));
     EL_HIP(c, hipEventRecord(e1, c->stream));
     std::vector<double> hm(M * 6);
@@ -846,6 +873,7 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     c->adjSumTex = gtex;
     c->stats.adjoint_ms = ms;
     c->stats.adjoint_rays = tally(h, kStatRays);
+    c->stats.adjoint_bytes = c->aCache.n + c->aImage.n + c->aMat.n + c->aTex.n;
     if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack overflow (gradients incomplete)");
     return EL_OK;
 }
@@ -1210,7 +1238,7 @@ static int batchEnsureWf(el_batch* b, size_t cap, size_t qcap) {
     EL_BHIP(b, b->wL.ensure(cap * 16));
     EL_BHIP(b, b->wPix.ensure(cap * 4));
     EL_BHIP(b, b->wSample.ensure(cap * 4));
-    EL_BHIP(b, b->wQuiet.ensure(cap * 4));
+    EL_BHIP(b, b->wQuiet.ensure(cap * 8));
     EL_BHIP(b, b->wCounters.ensure(kQCounters * sizeof(uint32_t)));
     b->wfCap = std::max(b->wfCap, cap);
     b->wfQcap = std::max(b->wfQcap, qcap);
@@ -1321,7 +1349,7 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
     q.pathL = b->wL.as<float4>();
     q.pathPix = b->wPix.as<uint32_t>();
     q.pathSample = b->wSample.as<uint32_t>();
-    q.pathQuiet = b->wQuiet.as<uint32_t>();
+    q.pathQuiet = b->wQuiet.as<uint64_t>();
     q.pathAov = nullptr;
     q.counters = b->wCounters.as<uint32_t>();
     q.segSize = (uint32_t)segSize;
@@ -1371,7 +1399,19 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
     const bool useGraph = (p->flags & EL_BATCH_GRAPH) != 0;
     if (useGraph) {
         // the captured sequence depends on the plan and the launch shape
+        // FNV-1a of the segment table and the generations: kernel arguments
+        // (segment counts, slot totals) are baked into the graph
+        uint64_t plan = 1469598103934665603ull;
+        auto mix = [&](const void* p, size_t n) {
+            for (size_t i = 0; i < n; i++) plan = (plan ^ ((const unsigned char*)p)[i]) * 1099511628211ull;
+        };
+        mix(segs.data(), segs.size() * sizeof(BatchSeg));
+        for (const BatchGen& g : gens) {
+            const uint64_t f[4] = {g.segOffset, g.nseg, g.count, g.finTotal};
+            mix(f, sizeof f);
+        }
         std::string key = std::to_string(S) + "/" + std::to_string(depth) + "/" + std::to_string(width) + "/" +
+                          std::to_string(p->flags & ~EL_BATCH_GRAPH) + "/" + std::to_string(plan) + "/" +
                           std::to_string(lc.block) + "/" + std::to_string(lc.refill) + "/" + std::to_string(Q) +
                           "/" + std::to_string(sppCap) + "/" + std::to_string(total) + "/" +
                           std::to_string((uintptr_t)b->dSegs.p) + "/" + std::to_string((uintptr_t)b->wRay0.p) +
@@ -1456,6 +1496,7 @@ extern "C" int el_batch_tune(el_batch* b, el_batch_params* io, uint32_t trial_sp
         int rc = batchRender(b, &p, trial_spp, &t);   // warm (BVH repack, buffers)
         if (rc) return rc;
         rc = batchRender(b, &p, trial_spp, &t);
+        if (rc) return rc;
         *ms = t;
         log << "width " << (p.bvh_width ? p.bvh_width : 4) << " queue " << p.queue_cap << " block "
             << (p.block ? p.block : kDefaultBlock) << " refill " << (p.refill ? p.refill : kDefaultRefill) << ": "

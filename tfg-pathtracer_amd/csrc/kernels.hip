@@ -333,12 +333,13 @@ __device__ __forceinline__ bool quietDeath(const DevScene& s, const PathRng& rng
 
 // Bit b set: bounce b's environment random number falls in a bucket that can
 // reach a pdf-0 / pdf-inf pixel (see quietDeath).  A path that dies at
-// bounce k stops quietly iff (mask >> (k + 1)) == 0.
-__device__ __forceinline__ uint32_t nanProneBounces(const DevScene& s, const PathRng& rng, int maxDepth) {
-    uint32_t m = 0;
-    for (int b = 0; b < maxDepth && b < 32; b++) {
+// bounce k < maxDepth - 1 stops quietly iff (mask >> (k + 1)) == 0; maxDepth
+// <= 64, so every bounce has its bit and the shift stays below 64.
+__device__ __forceinline__ uint64_t nanProneBounces(const DevScene& s, const PathRng& rng, int maxDepth) {
+    uint64_t m = 0;
+    for (int b = 0; b < maxDepth && b < 64; b++) {
         const uint32_t bucket = rng.block(2 + b).w >> 16;
-        m |= ((s.envBad[bucket >> 5] >> (bucket & 31)) & 1u) << b;
+        m |= (uint64_t)((s.envBad[bucket >> 5] >> (bucket & 31)) & 1u) << b;
     }
     return m;
 }
@@ -360,21 +361,40 @@ struct PathVertex {
     v3 Lincl;            // radiance gathered up to and including this vertex
 };
 
+// Where a walk's closest hits and shadow answers come from.
+//   kWalkTrace      traced.
+//   kWalkTraceKeep  traced; vertex i's closest hit (tri, t, u, v; tri -1:
+//                   escaped) is also written to io.keep[i * io.keepStride]
+//                   and the shadow answers to io.visOut (bit 2i HDRI, bit
+//                   2i + 1 point light, set when the light is visible).
+//   kWalkReplay     read from io.hits[i * io.stride] and io.vis, the record of
+//                   an earlier walk of the same path (a recording wavefront
+//                   render, or a kWalkTraceKeep walk); nothing is traced.  The
+//                   replay stops where the recorded walk did (both cull alike),
+//                   so every record read was written (a triangle index out of
+//                   range reads as an escape).
+constexpr int kWalkTrace = 0, kWalkTraceKeep = 1, kWalkReplay = 2;
+struct WalkIO {
+    const float4* hits;   // kWalkReplay
+    uint64_t stride;
+    uint32_t vis;
+    float4* keep;         // kWalkTraceKeep
+    uint64_t keepStride;
+    uint32_t visOut;
+};
+
 // One path of renderingKernel (kernel.cu:369-481) for pixel (x, y), sample
 // `sample`: camera ray, up to maxDepth vertices with DisneySample / shade and
 // both shadow queries, escape to the environment.  Returns the radiance
 // before the clamp; `hook` sees every shaded vertex; `verts` counts them.
-// REPLAY: the closest hits and shadow answers come from a recording
-// wavefront render of the same path (a.recHits / a.recVis) instead of being
-// traced; the path stops where the recording's did, so every record read was
-// written by it (a triangle index out of range reads as an escape).
-template <int W, bool REPLAY = false, class Hook>
+template <int W, int MODE = kWalkTrace, class Hook>
 __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, uint32_t x, uint32_t y,
                                        uint32_t idx, uint32_t sample, TravStack& st, TraceCounters& cnt,
-                                       uint32_t& rays, uint32_t& verts, Hook&& hook) {
+                                       uint32_t& rays, uint32_t& verts, Hook&& hook, WalkIO& io) {
+    constexpr bool REPLAY = MODE == kWalkReplay, KEEP = MODE == kWalkTraceKeep;
     PathRng rng{a.k0, a.k1, idx, sample, a.scene};
-    const uint64_t recAt = (uint64_t)sample * s.cam.xRes * s.cam.yRes + idx;
-    const uint32_t vis = REPLAY ? a.recVis[recAt] : 0u;
+    const uint32_t vis = REPLAY ? io.vis : 0u;
+    uint32_t visOut = 0;
     uint4 c = rng.block(0);
     uint32_t c5 = rng.block(1).x;
     v3 o, d;
@@ -384,7 +404,7 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
     for (i = 0; i < (int)a.maxDepth; i++) {
         TraceResult h;
         if constexpr (REPLAY) {
-            const float4 r = a.recHits[(uint64_t)i * a.recStride + recAt];
+            const float4 r = io.hits[(uint64_t)i * io.stride];
             h.tri = __float_as_int(r.x);
             h.t = r.y;
             h.u = r.z;
@@ -393,6 +413,7 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
         } else {
             rays++;
             h = traceRay<W, false, false>(s, o.x, o.y, o.z, d.x, d.y, d.z, __int_as_float(0x7f800000), st, cnt);
+            if constexpr (KEEP) io.keep[(uint64_t)i * io.keepStride] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
         }
         if (h.tri < 0) {
             L = L + envLookup(s, d) * T;
@@ -424,6 +445,7 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
             }
             if (visP) add = add + vo.contribP;
         }
+        if (KEEP) visOut |= (visH ? 1u : 0u) << (2 * i) | (visP ? 1u : 0u) << (2 * i + 1);
         L = L + add;
         hook(PathVertex{i, d, T, &sh, &hd, &vo, visH, visP, L});
         T = vo.newT;
@@ -434,8 +456,16 @@ __device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, u
             break;
         }
     }
+    if (KEEP) io.visOut = visOut;
     verts += (uint32_t)i;
     return L;
+}
+template <int W, class Hook>
+__device__ __forceinline__ v3 walkPath(const DevScene& s, const RenderArgs& a, uint32_t x, uint32_t y,
+                                       uint32_t idx, uint32_t sample, TravStack& st, TraceCounters& cnt,
+                                       uint32_t& rays, uint32_t& verts, Hook&& hook) {
+    WalkIO io{};
+    return walkPath<W, kWalkTrace>(s, a, x, y, idx, sample, st, cnt, rays, verts, hook, io);
 }
 
 template <int W>
@@ -588,7 +618,13 @@ __device__ __forceinline__ void waveAddGradients(int m, const float g[6], double
 //       + T w1 env cosH/hpdf dF_h [HDRI visible] + T w2 rad cosL/ppdf dF_l [light visible] ]
 // with a = dLoss/dI_pixel / accepted samples (0 on clamped channels) and
 // L_after = L - (radiance up to and including the vertex).  Nothing of the
-// path is stored: memory is the adjoint image, the counts and the gradients.
+// path outlives its thread: the primal walk keeps the closest hit of each of
+// its vertices and the shadow answers in the thread's own slots of `cache`
+// (vertex i at cache[i * gthreads + thread], coalesced across a wave; sized
+// by the resident threads and the depth, not by the pixels or samples), so
+// the replay recomputes every vertex from the RNG counters without tracing
+// again.  Memory is the adjoint image, the counts, the gradients and that
+// per-thread cache.
 // Per-vertex records (material, 6 gradients) are kept per lane (private
 // memory: they are indexed by the vertex number) until the path ends; the
 // wave then reduces them per material (ballot + shuffle sums) and one lane
@@ -596,13 +632,16 @@ __device__ __forceinline__ void waveAddGradients(int m, const float g[6], double
 // gradients of the selected texture go straight to FP32 atomics.
 constexpr int kAdjMaxDepth = kAdjMaxDepthHost;
 
-// REPLAY: the paths' vertices come from the recording render (walkPath), so
-// the replay traces nothing.
+// REPLAY: the paths' vertices come from a recording render (a.recHits /
+// a.recVis, EL_FLAG_RECORD_PATHS), so the kernel traces nothing.  Otherwise,
+// with `rec` (the recorded radiance of each path) one traced walk suffices;
+// without it the primal walk keeps its vertices in `cache` for the replay.
 template <int W, bool REPLAY>
 EL_KERNEL(REPLAY ? EL_WPE_ADJ_REPLAY : EL_WPE_ADJ) void k_adjoint(DevScene s, RenderArgs a, uint32_t spp, const float* __restrict__ dimage,
                                        const uint32_t* __restrict__ counts, const float4* __restrict__ rec,
-                                       int gradTex, double* __restrict__ dmat, float* __restrict__ dtex, int* gstack,
-                                       uint32_t gthreads, unsigned long long* counters) {
+                                       float4* __restrict__ cache, int gradTex, double* __restrict__ dmat,
+                                       float* __restrict__ dtex, int* gstack, uint32_t gthreads,
+                                       unsigned long long* counters) {
     TravStack st{};
     if constexpr (!REPLAY) st = makeStack(s, gstack, gthreads);
     TraceCounters cnt{0, 0};
@@ -611,6 +650,7 @@ EL_KERNEL(REPLAY ? EL_WPE_ADJ_REPLAY : EL_WPE_ADJ) void k_adjoint(DevScene s, Re
     const uint32_t perSample = tilesX * ((H_ + 15) / 16) * 256;
     const uint64_t total = (uint64_t)perSample * spp;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t rays = 0, verts = 0, paths = 0, overflow = 0;
     const uint32_t lane = __lane_id();
     // uniform trip count: every lane of a wave runs the same iterations
@@ -630,17 +670,21 @@ EL_KERNEL(REPLAY ? EL_WPE_ADJ_REPLAY : EL_WPE_ADJ) void k_adjoint(DevScene s, Re
                 const uint32_t n = counts[idx];
                 uint32_t vdummy = 0;
                 v3 Lt;
+                WalkIO io{};
                 if (REPLAY || rec) {
                     const float4 r = rec[(uint64_t)sample * W_ * H_ + idx];
                     Lt = mk(r.x, r.y, r.z);
                 } else if constexpr (!REPLAY) {
-                    Lt = walkPath<W>(s, a, x, y, idx, sample, st, cnt, rays, vdummy, [](const PathVertex&) {});
+                    io.keep = cache + gtid;
+                    io.keepStride = gthreads;
+                    Lt = walkPath<W, kWalkTraceKeep>(s, a, x, y, idx, sample, st, cnt, rays, vdummy,
+                                                     [](const PathVertex&) {}, io);
                 }
                 paths++;
                 float ad[3];
                 adjointWeight(Lt, dimage, idx, n, ad);
                 if (ad[0] != 0.f || ad[1] != 0.f || ad[2] != 0.f) {
-                    walkPath<W, REPLAY>(s, a, x, y, idx, sample, st, cnt, rays, verts, [&](const PathVertex& pv) {
+                    auto grad = [&](const PathVertex& pv) {
                         float g[6];
                         const int mat = vertexGradient(s, pv.d, pv.T, *pv.sh, *pv.hd, *pv.vo, pv.visH, pv.visP,
                                                        Lt - pv.Lincl, ad, gradTex, dtex, g);
@@ -650,7 +694,21 @@ EL_KERNEL(REPLAY ? EL_WPE_ADJ_REPLAY : EL_WPE_ADJ) void k_adjoint(DevScene s, Re
                             for (int q = 0; q < 6; q++) recG[nrec][q] = g[q];
                             nrec++;
                         }
-                    });
+                    };
+                    if constexpr (REPLAY) {
+                        const uint64_t at = (uint64_t)sample * W_ * H_ + idx;
+                        io.hits = a.recHits + at;
+                        io.stride = a.recStride;
+                        io.vis = a.recVis[at];
+                        walkPath<W, kWalkReplay>(s, a, x, y, idx, sample, st, cnt, rays, verts, grad, io);
+                    } else if (rec) {
+                        walkPath<W, kWalkTrace>(s, a, x, y, idx, sample, st, cnt, rays, verts, grad, io);
+                    } else {
+                        io.hits = cache + gtid;
+                        io.stride = gthreads;
+                        io.vis = io.visOut;
+                        walkPath<W, kWalkReplay>(s, a, x, y, idx, sample, st, cnt, rays, verts, grad, io);
+                    }
                 }
             }
         }
@@ -701,7 +759,7 @@ EL_KERNEL(1) void k_wf_camera(DevScene s, RenderArgs a, WfState q) {
             q.pathAov[3 * (uint64_t)slot + 2] = make_float4(0, 0, 0, 0);
         }
         v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
-        uint32_t quiet = 0xFFFFFFFFu;
+        uint64_t quiet = ~0ull;
         if (valid) {
             PathRng rng{a.k0, a.k1, idx, sample, a.scene};
             uint4 c = rng.block(0);
@@ -931,7 +989,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                     (front && dot(hd.normal, vo.hdir) > -1e-6f);
             wantP = vo.hasP && (vo.contribP.x != 0.f || vo.contribP.y != 0.f || vo.contribP.z != 0.f ||
                                 (front && dot(hd.normal, vo.ldir) > -1e-6f));
-            cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0u);
+            cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0ull);
             // only an extended path reads its throughput again
             if (cont) q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
         }
@@ -1089,7 +1147,7 @@ EL_KERNEL(1) void k_bt_camera(ManyScenes sc, const BatchSeg* __restrict__ segs, 
         q.pathPix[slot] = idx;   // 0xFFFFFFFF: padding; throughput and radiance: set by the first shade
         q.pathSample[slot] = sample;
         v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
-        uint32_t quiet = 0xFFFFFFFFu;
+        uint64_t quiet = ~0ull;
         if (valid) {
             PathRng rng{s.seedLo, s.seedHi, idx, sample, g.scene};
             uint4 c = rng.block(0);
@@ -1234,20 +1292,20 @@ hipError_t launchPlain(const LaunchCfg& c, const DevScene& s, const RenderArgs& 
 }
 
 hipError_t launchAdjoint(const LaunchCfg& c, const DevScene& s, const RenderArgs& a, uint32_t spp,
-                         const float* dimage, const uint32_t* counts, const float4* rec, int gradTex, double* dmat,
-                         float* dtex, unsigned long long* counters) {
+                         const float* dimage, const uint32_t* counts, const float4* rec, float4* cache, int gradTex,
+                         double* dmat, float* dtex, unsigned long long* counters) {
     uint64_t total = (uint64_t)((s.cam.xRes + 15) / 16) * ((s.cam.yRes + 15) / 16) * 256 * spp;
     uint32_t grid = gridFor(total, c.block, c.maxThreads / c.block);
     size_t lds = sizeof(int) * kShortStack * c.block;
     if (a.recHits)   // replays recorded vertices: no traversal, no stack
         hipLaunchKernelGGL((k_adjoint<4, true>), dim3(grid), dim3(c.block), 0, c.stream, s, a, spp, dimage, counts,
-                           rec, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
+                           rec, cache, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
     else if (s.width == 4)
         hipLaunchKernelGGL((k_adjoint<4, false>), dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts,
-                           rec, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
+                           rec, cache, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
     else
         hipLaunchKernelGGL((k_adjoint<8, false>), dim3(grid), dim3(c.block), lds, c.stream, s, a, spp, dimage, counts,
-                           rec, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
+                           rec, cache, gradTex, dmat, dtex, c.gstack, c.maxThreads, counters);
     return hipGetLastError();
 }
 

@@ -71,7 +71,7 @@ struct WfState {
     float4* pathL;          // radiance per path slot (from the first shade on)
     uint32_t* pathPix;      // batches: output pixel index per path slot, 0xFFFFFFFF: padding
     uint32_t* pathSample;   // batches: sample index per path slot (one scene: slotPixel)
-    uint32_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b)
+    uint64_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b, b < 64)
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
     uint32_t* pathVis;      // recording renders: visibility bits per path slot (RenderArgs::recVis), else null
     uint32_t visBit;        // shadow launches of a recording render: the bit a visible query sets
@@ -108,15 +108,17 @@ hipError_t launchTrace(const LaunchCfg& c, const dev::DevScene& s, const float* 
                        unsigned long long* counters);
 hipError_t launchPlain(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, float4* accum,
                        uint32_t* counts, float4* aov, unsigned long long* counters);
-// rec: the forward render's per-path radiance (EL_FLAG_RECORD_PATHS) or null
-// (then every path is walked once for its radiance, once for the gradient).
 // r = 2 (mean image - target) per pixel and channel (FP64 arithmetic, FP32
 // store); *loss += sum (mean image - target)^2 (the caller zeroes it).
 hipError_t launchResidualL2(hipStream_t stream, const float4* accum, const uint32_t* counts, const float* target,
                             uint32_t npx, float* r, double* loss);
+// rec: the forward render's per-path radiance (EL_FLAG_RECORD_PATHS) or null:
+// then every path is walked once for its radiance, keeping its vertices in
+// `cache` (c.maxThreads * a.maxDepth float4, vertex i of grid thread g at
+// i * c.maxThreads + g), and replayed from there for the gradient.
 hipError_t launchAdjoint(const LaunchCfg& c, const dev::DevScene& s, const RenderArgs& a, uint32_t spp,
-                         const float* dimage, const uint32_t* counts, const float4* rec, int gradTex, double* dmat,
-                         float* dtex, unsigned long long* counters);
+                         const float* dimage, const uint32_t* counts, const float4* rec, float4* cache, int gradTex,
+                         double* dmat, float* dtex, unsigned long long* counters);
 // Batch (many scenes, scene id per queue entry): scenes = device array of
 // DevScene; segs = the generation's segment table (device).
 hipError_t launchBtCamera(const LaunchCfg& c, const dev::DevScene* scenes, const BatchSeg* segs, uint32_t nseg,

@@ -93,7 +93,8 @@ class el_stats(C.Structure):
                 ("node_fetches", C.c_uint64), ("tri_tests", C.c_uint64),
                 ("node_bytes", C.c_uint64), ("tri_bytes", C.c_uint64), ("bvh_width", C.c_uint32),
                 ("stack_bound", C.c_uint32), ("node_stride", C.c_uint32), ("tri_stride", C.c_uint32),
-                ("adjoint_ms", C.c_double), ("adjoint_rays", C.c_uint64), ("texels_8bit", C.c_uint64)]
+                ("adjoint_ms", C.c_double), ("adjoint_rays", C.c_uint64), ("texels_8bit", C.c_uint64),
+                ("adjoint_bytes", C.c_uint64), ("record_bytes", C.c_uint64)]
 
 
 class el_adjoint_params(C.Structure):

@@ -12,6 +12,13 @@ a backward must follow its forward before the context renders again.
 The renderer is deterministic for a fixed seed (Philox counters per pixel,
 sample and bounce), so forward calls with the same seed use common random
 numbers: finite differences of the loss are differences of the same paths.
+
+Memory: by default nothing of a path is kept between the forward and the
+backward; the adjoint recomputes every path from its RNG counters (path
+replay), so the device memory of a gradient is O(pixels).  record=True keeps
+each path's radiance and, in wavefront mode, every vertex (16 B per path +
+16 B per vertex: O(pixels x spp x depth)), which lets the adjoint walk each
+path once without tracing.
 """
 from __future__ import annotations
 
@@ -49,10 +56,10 @@ class Renderer:
     """Differentiable forward render of one uploaded scene."""
 
     def __init__(self, ctx: Context, scene: SceneData, spp: int, depth: int = 5, seed: int = 0,
-                 grad_texture: int = -1, mode: int = MODE_WAVEFRONT):
+                 grad_texture: int = -1, mode: int = MODE_WAVEFRONT, record: bool = False):
         self.ctx, self.scene = ctx, scene
         self.spp, self.depth, self.seed = spp, depth, seed
-        self.grad_texture, self.mode = grad_texture, mode
+        self.grad_texture, self.mode, self.record = grad_texture, mode, record
         self.base = scene.materials()
 
     def render(self, params: np.ndarray, texture: np.ndarray | None = None, read: bool = True) -> np.ndarray | None:
@@ -62,9 +69,10 @@ class Renderer:
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
         self.ctx.clear()
-        # the recorded paths (radiance; wavefront: also every vertex) let the
+        # record: the paths' radiance (wavefront: also every vertex) let the
         # backward replay each path once, without tracing in wavefront mode
-        self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode, flags=FLAG_RECORD_PATHS)
+        self.ctx.render(self.spp, depth=self.depth, seed=self.seed, mode=self.mode,
+                        flags=FLAG_RECORD_PATHS if self.record else 0)
         return self.ctx.read()[0][..., :3].copy() if read else None
 
     def __call__(self, params: torch.Tensor, texture: torch.Tensor | None = None) -> torch.Tensor:

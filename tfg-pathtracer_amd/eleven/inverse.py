@@ -17,6 +17,10 @@ With decorrelate=False the gradient is the exact derivative of the measured
 loss of one render (what a finite-difference check with common random
 numbers measures).  The parameters and texels are clamped to their valid
 ranges after every Adam update.
+
+Memory: by default the adjoint replays each path from its RNG counters
+(O(pixels) device memory per view); record=True keeps the replayed render's
+paths instead (EL_FLAG_RECORD_PATHS: O(pixels x spp x depth)).
 """
 from __future__ import annotations
 
@@ -56,9 +60,10 @@ class MultiView:
     seen from several cameras."""
 
     def __init__(self, ctx: Context, scene: SceneData, cameras: list, spp: int, depth: int = 6,
-                 grad_texture: int = -1, mode: int = MODE_WAVEFRONT):
+                 grad_texture: int = -1, mode: int = MODE_WAVEFRONT, record: bool = False):
         self.ctx, self.scene, self.cameras = ctx, scene, list(cameras)
         self.spp, self.depth, self.grad_texture, self.mode = spp, depth, grad_texture, mode
+        self.record = record
         self.base = scene.materials()
 
     def set_params(self, params: np.ndarray, texture: np.ndarray | None = None):
@@ -88,12 +93,12 @@ class MultiView:
         dmat = dtex = None
         for v in range(nv):
             # the residual 2 (I - target) is formed on the device and stays there
-            self.render_view(v, seed + v, record=not decorrelate, read=False)
+            self.render_view(v, seed + v, record=self.record and not decorrelate, read=False)
             loss += self.ctx.residual_l2(targets[v])
             if decorrelate:   # the paths the adjoint replays (same view: the residual stays valid)
                 self.ctx.clear()
                 self.ctx.render(self.spp, depth=self.depth, seed=seed + v + DECORRELATE_SEED, mode=self.mode,
-                                flags=FLAG_RECORD_PATHS)
+                                flags=FLAG_RECORD_PATHS if self.record else 0)
             # gradients summed over the views on the device, read once
             dmat, dtex = self.ctx.adjoint(None, grad_texture=self.grad_texture if texture is not None else -1,
                                           accumulate=v > 0, read=v == nv - 1)
