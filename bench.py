@@ -65,25 +65,39 @@ def measured_traffic(kernel="k_wf_extend"):
 def copy_bandwidth(gib=2, reps=5):
     """Achievable device bandwidth on this GPU: the better of a device-to-device
     copy and an elementwise copy kernel over `gib` GiB (bytes read + written
-    per second), timed with events on torch's current stream."""
+    per second), timed with events on torch's current stream; and the same
+    copy over buffers that stay cache-resident between repetitions: 1 MiB
+    read + 1 MiB written (L2: 4 MB per XCD) and the traversal's working sets
+    (the heightfield's 22.4 MB node array, nodes + triangles 72.7 MB: the
+    256 MB MALL).  Returns {"copy_gbs": HBM copy, "resident_gbs": {MiB: GB/s}}."""
     import torch
-    n = (gib << 30) // 4
-    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
-    y = torch.empty_like(x)
-    best = 0.0
-    for op in (lambda: y.copy_(x), lambda: torch.mul(x, 1.0, out=y)):
+
+    def timed(op, nbytes, n):
         op()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
-        for _ in range(reps):
+        for _ in range(n):
             op()
         e1.record()
         e1.synchronize()
-        best = max(best, 2 * n * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+        return nbytes * n / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+    n = (gib << 30) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    best = max(timed(lambda: y.copy_(x), 2 * n * 4, reps), timed(lambda: torch.mul(x, 1.0, out=y), 2 * n * 4, reps))
     del x, y
+    resident = {}
+    for mib in (1, 22.4, 72.7):
+        m = int(mib * (1 << 20)) // 4
+        a = torch.ones(m, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        k = max(200, int(20e9 / (8 * m)))
+        resident[str(mib)] = round(timed(lambda: b.copy_(a), 2 * m * 4, k), 1)
+        del a, b
     torch.cuda.empty_cache()
-    return best
+    return {"copy_gbs": best, "resident_gbs": resident}
 
 
 def cpu_threads():
@@ -104,32 +118,159 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(scene, budget_s=15.0, threads=None):
-    """Oracle render (FP32, reference estimator) of a centred crop of the frame,
-    its sample count chosen from a short probe so the timed part takes about
-    `budget_s` seconds on `threads` host cores."""
-    sys.path.insert(0, str(ROOT / "tests"))
-    import oracle_py as orc
-    threads = threads or cpu_threads()
-    cam = scene.camera
+_ORACLE = None
+
+
+def native_oracle():
+    """The CPU oracle compiled for this host (-O3 -march=native, one
+    std::thread per core) into a temporary directory, for the CPU baselines
+    only; FP contraction stays off, so it computes what the oracle the tests
+    use computes."""
+    global _ORACLE
+    if _ORACLE is None:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import tempfile
+        import oracle_py as orc
+        so = Path(tempfile.gettempdir()) / f"eleven_oracle_native_{os.getpid()}" / "liboracle.so"
+        so.parent.mkdir(parents=True, exist_ok=True)
+        subprocess.run(["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+                        "-shared", "-o", str(so), str(ROOT / "oracle" / "oracle.cpp"), "-lpthread"], check=True)
+        orc.ORACLE_SO = so
+        _ORACLE = orc
+    return _ORACLE
+
+
+def cpu_render(scene, depth, window, budget_s, threads, seed=0):
+    """The oracle's FP32 estimator (the reference's arithmetic) over the
+    reference-faithful depth-18 / 14-bin SAH binary BVH on `window`, its
+    sample count chosen from a 1-spp probe so the timed part takes about
+    budget_s.  Returns (Mrays/s, spp, rays, seconds, tree build seconds)."""
+    orc = native_oracle()
     t0 = time.perf_counter()
     ref = orc.RefBVH(scene.vertices())
     build_s = time.perf_counter() - t0
-    w, h = max(1, cam.x_res // 8), max(1, cam.y_res // 8)
-    x0, y0 = cam.x_res // 2 - w // 2, cam.y_res // 2 - h // 2
-    win = (x0, x0 + w, y0, y0 + h)
     t0 = time.perf_counter()
-    orc.render(scene, ref, 1, depth=8, seed=0, window=win, threads=threads)
+    orc.render(scene, ref, 1, depth=depth, seed=seed, window=window, threads=threads)
     probe = max(time.perf_counter() - t0, 1e-3)
     spp = max(1, min(4096, int(budget_s / probe)))
     t0 = time.perf_counter()
-    r = orc.render(scene, ref, spp, depth=8, seed=0, s0=1, window=win, threads=threads)
+    r = orc.render(scene, ref, spp, depth=depth, seed=seed, s0=1, window=window, threads=threads)
     dt = time.perf_counter() - t0
-    rays = r[3]
-    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+    return r[3] / dt / 1e6, spp, r[3], dt, build_s
+
+
+def cpu_baseline(scene, budget_s=10.0, threads=None):
+    """CPU baseline of the headline workload: the oracle's FP32 estimator
+    (the reference's arithmetic) on a centred crop of the same frame."""
+    threads = threads or cpu_threads()
+    cam = scene.camera
+    w, h = max(1, cam.x_res // 8), max(1, cam.y_res // 8)
+    x0, y0 = cam.x_res // 2 - w // 2, cam.y_res // 2 - h // 2
+    mr, spp, rays, dt, build_s = cpu_render(scene, 8, (x0, x0 + w, y0, y0 + h), budget_s, threads)
+    return {"value": round(mr, 3), "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "build": "g++ -O3 -march=native, one std::thread per core",
             "sample": f"{w}x{h} centre crop of the same frame, {spp} spp, depth 8, {rays} rays in {dt:.1f} s "
                       f"(FP32 restatement of the reference estimator over the reference-faithful depth-18 / "
                       f"14-bin SAH binary BVH; tree build {build_s:.1f} s not timed)"}
+
+
+def c0_bench(with_cpu=True, reps=20):
+    """c0, the CPU-runnable forward configuration: the 64-sphere field
+    (61,442 tris) at 256x256, 16 spp, depth 5, constant environment 0.5, 2
+    point lights.  Wavefront and plain renders of the frame (rays credited
+    as for the heightfield: the faithful query count), their relMSE against
+    the 16,384-spp FP64-accumulated oracle reference
+    (tests/golden/ref_c0_sphere_field_256.npz) beside the relMSE the sample
+    variance predicts at 16 spp, the traversal alone on 2^20 random rays +
+    the frame's 256x256x16 camera rays (GPU closest hit vs the
+    reference-faithful CPU traversal, BVH.hpp:120-157), and the CPU
+    render baseline of the whole frame."""
+    import eleven
+    import numpy as np
+    from eleven import scenegen
+    sys.path.insert(0, str(ROOT / "tests"))
+    import raygen
+    d = scenegen.cached("sphere_field", n_spheres=64, res=(256, 256), seed=0)
+    scene = eleven.SceneData.load(d)
+    ctx = eleven.Context(0)
+    ctx.upload(scene)
+    spp, depth = 16, 5
+    ctx.clear()
+    logical = ctx.render(spp, depth=depth, seed=0, flags=eleven.FLAG_NO_CULL).rays
+    out = {"metric": "Mrays/s", "unit": "Mrays/s",
+           "config": "sphere field 61,442 tris, 256x256, 16 spp, depth 5, constant environment 0.5, 2 point lights",
+           "rays_per_frame": logical}
+    ref = np.load(ROOT / "tests" / "golden" / "ref_c0_sphere_field_256.npz")
+    R, V = ref["mean"].astype(np.float64), ref["var"].astype(np.float64)
+    w = 1.0 / (R ** 2 + 1e-2)
+    for name, mode in (("wavefront", eleven.MODE_WAVEFRONT), ("plain", eleven.MODE_PLAIN)):
+        ctx.clear()
+        ctx.render(spp, depth=depth, seed=0, mode=mode)                 # warm
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.clear()
+            ctx.render(spp, depth=depth, seed=0, mode=mode)
+        wall = (time.perf_counter() - t0) / reps
+        img, cnt, _, _ = ctx.read()
+        n = np.maximum(cnt, 1)[..., None].astype(np.float64)
+        rel = float(np.mean((img[..., :3] - R) ** 2 * w))
+        pred = float(np.mean((V / n + V / ref["count"][..., None]) * w))
+        out[name] = {"value": round(logical / wall / 1e6, 2), "ms_per_frame": round(wall * 1e3, 3),
+                     "relmse": rel, "relmse_predicted": pred}
+    out["value"] = out["wavefront"]["value"]
+    out["wavefront_vs_plain"] = round(out["wavefront"]["value"] / out["plain"]["value"], 3)
+    # traversal alone: the same ray batch on the GPU and the CPU
+    rays = np.concatenate([raygen.random_rays(scene.vertices(), 1 << 20, seed=1),
+                           raygen.camera_rays(scene.camera, spp=16, seed=2)])
+    n = rays.shape[0]
+    dr, dtri, dt = ctx.alloc(rays.nbytes), ctx.alloc(4 * n), ctx.alloc(4 * n)
+    ctx.htod(dr, rays)
+    ms = ctx.trace_bench(dr, n, dtri, dt, 20)
+    for p_ in (dr, dtri, dt):
+        ctx.free(p_)
+    out["traversal"] = {"rays": n, "gpu_mrays": round(n / (ms * 1e-3) / 1e6, 1), "gpu_ms": round(ms, 4),
+                        "batch": "2^20 random rays (origins in the scene box grown 20 %, uniform directions) + "
+                                 "the 256x256x16 camera rays"}
+    ctx.close()
+    if with_cpu:
+        threads = cpu_threads()
+        orc = native_oracle()
+        rb = orc.RefBVH(scene.vertices())
+        rb.closest(rays[:4096], threads=threads)
+        t0 = time.perf_counter()
+        rb.closest(rays, threads=threads)
+        dt = time.perf_counter() - t0
+        out["traversal"]["cpu_baseline"] = {
+            "value": round(n / dt / 1e6, 2), "unit": "Mrays/s", "cores": threads, "kind": "port", "cpu": cpu_model(),
+            "sample": f"the same {n} rays through the reference-faithful depth-18 / 14-bin SAH binary BVH "
+                      f"(44 B nodes, BVH.hpp:120-157, closest by t) in {dt:.2f} s"}
+        mr, cspp, crays, cdt, _ = cpu_render(scene, depth, (0, 256, 0, 256), 10.0, threads)
+        out["cpu_baseline"] = {"value": round(mr, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                               "cpu": cpu_model(), "build": "g++ -O3 -march=native, one std::thread per core",
+                               "sample": f"the full 256x256 frame at {cspp} spp, depth 5, {crays} rays in "
+                                         f"{cdt:.1f} s (FP32 estimator over the reference-faithful BVH)"}
+    return out
+
+
+def crop_relmse(ctx, scene, args):
+    """relMSE of the last timed frame on the three fixed 64x64 crops against
+    their 16,384-spp FP64-accumulated oracle references
+    (tests/golden/ref_c1_heightfield_crops.npz), beside the relMSE the
+    per-sample variance predicts at the frame's sample count."""
+    import numpy as np
+    ref = np.load(ROOT / "tests" / "golden" / "ref_c1_heightfield_crops.npz")
+    img, cnt, _, _ = ctx.read()
+    H = scene.camera.y_res
+    meas, pred = [], []
+    for k, (x0, x1, y0, y1) in enumerate(ref["windows"]):
+        g = img[H - y1:H - y0, x0:x1, :3].astype(np.float64)
+        n = np.maximum(cnt[H - y1:H - y0, x0:x1], 1)[..., None].astype(np.float64)
+        R, V = ref["mean"][k].astype(np.float64), ref["var"][k].astype(np.float64)
+        w = 1.0 / (R ** 2 + 1e-2)
+        meas.append(float(np.mean((g - R) ** 2 * w)))
+        pred.append(float(np.mean((V / n + V / ref["count"][k][..., None]) * w)))
+    return {"measured": [round(x, 6) for x in meas], "predicted": [round(x, 6) for x in pred],
+            "windows": ref["windows"].tolist(), "reference": f"{int(ref['spp'])}-spp oracle (FP32 shading, FP64 sums)"}
 
 
 def adjoint_bench(steps=5, with_cpu=True):
@@ -191,8 +332,7 @@ def adjoint_bench(steps=5, with_cpu=True):
                                                 "and vertices (O(pixels x spp x depth)); the adjoint traces nothing"}}
     ctx.close()
     if with_cpu:
-        sys.path.insert(0, str(ROOT / "tests"))
-        import oracle_py as orc
+        orc = native_oracle()
         threads = cpu_threads()
         rb = orc.RefBVH(scene.vertices())
         w = (2 * (img - target)).astype(np.float32)
@@ -203,6 +343,7 @@ def adjoint_bench(steps=5, with_cpu=True):
             n += 1
         dt = time.perf_counter() - t0c
         out["cpu_baseline"] = {"value": round(n / dt, 4), "unit": "grads/s", "cores": threads, "kind": "port",
+                               "cpu": cpu_model(), "build": "g++ -O3 -march=native, one std::thread per core",
                                "sample": f"FP64 oracle adjoint (primal walk + replay) of the same loss and "
                                          f"configuration, {n} gradients in {dt:.2f} s"}
     return out
@@ -340,6 +481,7 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--save", default="")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-c0", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--inverse-steps", type=int, default=200)
@@ -352,7 +494,7 @@ def main():
     args = ap.parse_args()
 
     if args.copy_bw:
-        print(json.dumps({"copy_gbs": copy_bandwidth()}))
+        print(json.dumps(copy_bandwidth()))
         return
 
     if args.copy_calib:
@@ -373,14 +515,15 @@ def main():
 
     # torch and the library each load a HIP runtime: the copy kernel runs in
     # its own process, before this one initialises the GPU
-    copy_gbs = None
+    copy_gbs, resident = None, {}
     if args.mode == "wavefront" and not args.no_count:
         r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--copy-bw"], capture_output=True, text=True,
                            timeout=300)
         if r.returncode != 0:
             raise RuntimeError(f"copy bandwidth run failed: {r.stderr[-2000:]}")
-        copy_gbs = json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"]
-        log(f"copy kernel: {copy_gbs:.0f} GB/s")
+        cb = json.loads(r.stdout.strip().splitlines()[-1])
+        copy_gbs, resident = cb["copy_gbs"], cb["resident_gbs"]
+        log(f"copy kernel: {copy_gbs:.0f} GB/s; cache-resident copies {resident}")
 
     ensure_built()
     import eleven
@@ -416,7 +559,7 @@ def main():
     log(f"faithful (no culling): {sf.render_ms:.1f} ms, {logical} rays")
     traced = 0
     ext_ms = 0.0
-    ext_launches = 0
+    ext_launches = ext_rays = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         ctx.clear()
@@ -424,11 +567,13 @@ def main():
         traced += st.rays
         ext_ms += st.extend_ms
         ext_launches += st.extend_launches
+        ext_rays += st.extend_rays
         log(f"step {k}: {st.render_ms:.1f} ms, {st.rays} rays traced")
     wall = time.perf_counter() - t0
     rays = logical * args.steps
     value = rays / wall / 1e6
     st = ctx.stats()
+    crops = crop_relmse(ctx, scene, args) if (args.res, args.depth) == ("1920x1080", 8) else None
     width = st.bvh_width
     if args.save:
         from eleven import imageio
@@ -455,27 +600,40 @@ def main():
         sc = ctx.render(args.spp, seed=0, flags=eleven.FLAG_COUNT_TRAVERSAL, **kw)
         per_frame_launches = sc.extend_launches
         s_node, s_tri = sc.node_stride, sc.tri_stride   # bytes per node / triangle record as stored
-        algo_bytes = sc.node_fetches * s_node + sc.tri_tests * s_tri
-        bytes_per_launch = algo_bytes / max(1, per_frame_launches)
         avg_ms = ext_ms / ext_launches
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        ext_rays = sc.rays - 0  # informational
+        rays_per_launch = ext_rays / ext_launches
+        # algorithmic HBM bytes of one launch: its queue entries read (32 B)
+        # and hit records written (16 B) per ray, plus the node array and the
+        # triangle stream read once
+        algo = rays_per_launch * (32 + 16) + st.node_bytes + st.tri_bytes
+        achieved = algo / (avg_ms * 1e-3) / 1e9
+        # the bytes the traversal visits (every node fetch and triangle
+        # test), served mostly by the caches: against cache-resident copies
+        visit = (sc.node_fetches * s_node + sc.tri_tests * s_tri) / max(1, per_frame_launches)
+        visit_gbs = visit / (avg_ms * 1e-3) / 1e9
+        ws_mib = (st.node_bytes + st.tri_bytes) / 2 ** 20
         defaults = (args.block, args.queue, args.refill, args.spp, args.depth, args.res) == (0, 0, 0, 64, 8, "1920x1080")
         traffic = measured_traffic() if defaults and width == 4 else None
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "traffic_over_algorithmic": round(traffic / algo, 2) if traffic else None,
+                           "traffic_gbs": round(traffic / (avg_ms * 1e-3) / 1e9, 1) if traffic else None,
+                           "traffic_frac": round(traffic / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None,
                            "copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
-                           "frac_of_copy": round(achieved / copy_gbs, 4) if copy_gbs else None,
                            "traffic_source": "profiles/hbm_traffic_heightfield_1080p_64spp.md" if traffic else None,
                            "kernel": "k_wf_extend (closest-hit traversal)",
                            "avg_launch_ms": round(avg_ms, 4), "launches_per_step": per_frame_launches,
-                           "algo_bytes_per_launch": int(bytes_per_launch),
-                           "node_fetches_per_step": sc.node_fetches, "tri_tests_per_step": sc.tri_tests,
-                           "bytes_per_node": s_node, "bytes_per_tri": s_tri}
+                           "rays_per_launch": int(rays_per_launch), "algo_bytes_per_launch": int(algo),
+                           "cache": {"visit_bytes_per_launch": int(visit), "achieved_gbs": round(visit_gbs, 1),
+                                     "resident_copy_gbs": resident,
+                                     "working_set_mib": round(ws_mib, 1),
+                                     "frac_of_working_set_copy": round(visit_gbs / resident["72.7"], 4)
+                                     if resident else None,
+                                     "node_fetches_per_step": sc.node_fetches, "tri_tests_per_step": sc.tri_tests,
+                                     "bytes_per_node": s_node, "bytes_per_tri": s_tri}}
         out["config"]["extend_ms_per_step"] = round(ext_ms / args.steps, 2)
         out["config"]["shade_ms_per_step"] = round(st.shade_ms, 2)
         out["config"]["shadow_ms_per_step"] = round(st.shadow_ms, 2)
-        del ext_rays
 
     if not args.no_plain and mode == eleven.MODE_WAVEFRONT:
         # the reference's structure: one thread per pixel, every path traced to the end
@@ -493,10 +651,15 @@ def main():
                         "with_cull": {"value": round(sp.rays / pwc / 1e6, 2), "ms_per_step": round(pwc * 1e3, 2),
                                       "rays_traced": spc.rays}}
 
+    if crops:
+        out["config"]["crop_relmse"] = crops
     if not args.no_cpu:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene)
     ctx.close()
+    if not args.no_c0:
+        log("c0 ...")
+        out["c0"] = c0_bench(with_cpu=not args.no_cpu)
     if not args.no_adjoint:
         log("adjoint ...")
         out["adjoint"] = adjoint_bench(with_cpu=not args.no_cpu)

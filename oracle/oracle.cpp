@@ -946,11 +946,24 @@ struct NoHook {
 };
 
 // One path (kernel.cu:369-481).  Returns true if the sample is accepted.
+// aov9 (optional): the first hit's normal, tangent and bitangent
+// (kernel.cu:435-440; zero if the camera ray escapes).
 template <class R>
 bool tracePath(const SceneC& s, const or_render_params& p, int x, int y, uint32_t sample, V3<R>& Lout,
-               uint64_t& rays, uint64_t& verts) {
+               uint64_t& rays, uint64_t& verts, double* aov9 = nullptr) {
     V3<R> L;
-    walkPath<R>(s, p, x, y, sample, L, rays, verts, NoHook());
+    if (aov9)
+        for (int k = 0; k < 9; k++) aov9[k] = 0;
+    walkPath<R>(s, p, x, y, sample, L, rays, verts, [&](const VertexRec<R>& rec) {
+        if (aov9 && rec.depth == 0) {
+            const V3<R> a[3] = {rec.sh.normal, rec.sh.tangent, rec.sh.bitangent};
+            for (int k = 0; k < 3; k++) {
+                aov9[3 * k] = (double)a[k].x;
+                aov9[3 * k + 1] = (double)a[k].y;
+                aov9[3 * k + 2] = (double)a[k].z;
+            }
+        }
+    });
     L = V3<R>(clampR<R>(L.x, 0, 10), clampR<R>(L.y, 0, 10), clampR<R>(L.z, 0, 10));
     if (std::isnan(L.x) || std::isnan(L.y) || std::isnan(L.z)) return false;
     Lout = L;
@@ -966,16 +979,18 @@ void renderT(const SceneC& s, const or_render_params& p, double* sum, double* su
         int x = p.x0 + k % nx, y = p.y0 + k / nx;
         uint32_t idx = cam.x_res * (cam.y_res - y - 1) + x;
         uint64_t lr = 0, lv = 0;
-        double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+        double a[3] = {0, 0, 0}, b[3] = {0, 0, 0}, ao[9] = {0}, a9[9];
         uint32_t n = 0;
         for (uint32_t sidx = 0; sidx < p.spp; sidx++) {
             V3<R> L;
-            if (tracePath<R>(s, p, x, y, p.s0 + sidx, L, lr, lv)) {
+            if (tracePath<R>(s, p, x, y, p.s0 + sidx, L, lr, lv, p.aov ? a9 : nullptr)) {
                 double c3[3] = {(double)L.x, (double)L.y, (double)L.z};
                 for (int j = 0; j < 3; j++) {
                     a[j] += c3[j];
                     b[j] += c3[j] * c3[j];
                 }
+                if (p.aov)
+                    for (int j = 0; j < 9; j++) ao[j] += a9[j];
                 n++;
             }
         }
@@ -983,6 +998,8 @@ void renderT(const SceneC& s, const or_render_params& p, double* sum, double* su
             sum[3 * (size_t)idx + j] += a[j];
             sumsq[3 * (size_t)idx + j] += b[j];
         }
+        if (p.aov)
+            for (int j = 0; j < 9; j++) p.aov[9 * (size_t)idx + j] += ao[j];
         count[idx] += n;
         rays += lr;
         verts += lv;

@@ -5,11 +5,15 @@
  * loads or calls it.
  *
  * Every function restates the reference (src/tfg-pathtracer/, read as text)
- * and cites the lines it follows.  Nothing here is copied from the reference;
- * the reference cannot be built in this image (its headers need the CUDA
- * runtime header, which is absent, and stand-ins are not allowed), so the
- * oracle is pinned by closed forms, brute force, invariants and exhaustive
- * checks in tests/test_oracle_*.py.
+ * and cites the lines it follows.  Nothing here is copied from the reference.
+ * The reference's estimator cannot be built in this image (kernel.cu needs
+ * nvcc and cuRAND, and its host headers need the CUDA runtime header, which
+ * is absent; stand-ins are not allowed), so the oracle is pinned by closed
+ * forms, brute force, invariants, exhaustive checks and exact expectations
+ * in tests/test_oracle_*.py.  The reference's vendored MikkTSpace (plain C)
+ * does build where it lies: oracle/Makefile compiles it into oracle/_ref/
+ * behind mikk_ref_glue.cpp, and tests/test_tangents.py pins the loader's
+ * tangents against it.
  *
  * Scene input uses the product's plain-C el_scene_view struct (a POD layout
  * shared with include/eleven.h; no product code is called).
@@ -89,6 +93,10 @@ typedef struct {
      * pdfs and MIS weights keep the scene's: the detached-sampling estimator
      * whose derivative or_adjoint computes (test use only). */
     const el_material* eval_materials;
+    /* NULL, or [W*H][9] (output pixel idx): sums over the accepted samples of
+     * the first hit's normal, tangent and bitangent (the NORMAL / TANGENT /
+     * BITANGENT passes, kernel.cu:435-440, 451-476). */
+    double* aov;
 } or_render_params;
 
 int or_render(const el_scene_view* scene, void* refbvh, const or_render_params* p, double* sum, double* sumsq,

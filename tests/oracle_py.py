@@ -26,7 +26,7 @@ class or_render_params(C.Structure):
     _fields_ = [("spp", C.c_uint32), ("s0", C.c_uint32), ("depth", C.c_uint32), ("seed", C.c_uint64),
                 ("precision", C.c_int), ("shadow_mode", C.c_int), ("threads", C.c_int),
                 ("x0", C.c_int), ("x1", C.c_int), ("y0", C.c_int), ("y1", C.c_int),
-                ("eval_materials", C.c_void_p)]
+                ("eval_materials", C.c_void_p), ("aov", C.c_void_p)]
 
 
 def lib():
@@ -223,11 +223,13 @@ def camera_ray(cam, x, y, r5):
 
 
 def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1, threads=0, window=None,
-           eval_materials=None):
+           eval_materials=None, aov=None):
     """Oracle render of pixels `window` = (x0, x1, y0, y1) (kernel x, y).
 
     eval_materials: optional ctypes array of el_material whose BRDF values
     are used while sampling, pdfs and MIS weights keep the scene's.
+    aov: optional float64 array [H, W, 9] that receives the sums of the first
+    hit's normal, tangent and bitangent over the accepted samples.
     Returns (sum[H,W,3], sumsq[H,W,3], count[H,W], rays, vertices) in output
     layout (row 0 = top)."""
     cam = scene.view.camera
@@ -238,6 +240,9 @@ def render(scene, refbvh, spp, depth=5, seed=0, s0=0, precision=0, shadow_mode=1
     p.precision, p.shadow_mode, p.threads = precision, shadow_mode, threads
     p.x0, p.x1, p.y0, p.y1 = x0, x1, y0, y1
     p.eval_materials = C.cast(eval_materials, C.c_void_p) if eval_materials is not None else None
+    if aov is not None:
+        assert aov.dtype == np.float64 and aov.shape == (H, W, 9) and aov.flags.c_contiguous
+        p.aov = aov.ctypes.data
     s = np.zeros((H, W, 3))
     ss = np.zeros((H, W, 3))
     cnt = np.zeros((H, W), np.uint32)

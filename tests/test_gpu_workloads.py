@@ -1,0 +1,345 @@
+"""GPU parity on the benchmark workloads themselves (not miniatures).
+
+c0  64-sphere field, 61,442 tris, 256x256, depth 5: the full frame of the
+    wavefront and the plain renderer (1024 spp each) against the 16,384-spp
+    FP64-accumulated oracle reference (FP32 shading as the reference,
+    tests/golden/ref_c0_sphere_field_256.npz); both consistent with it at 3
+    sigma (imagestats): equal relMSE.
+c1  heightfield, 1,048,576 tris, 1920x1080, depth 8, defocus, 16 textured
+    materials (1024^2 8-bit maps), 8 lights, 2048x1024 HDR sky: the three
+    fixed 64x64 crops of a 1024-spp frame against the 16,384-spp crop
+    references, and the same crops against the oracle with common random
+    numbers (same seed, same Philox streams).
+c2  the 32-scene batch (1k-500k tris each, 128-1024 px, 4-64 spp, depth 6,
+    a 512x256 sky each), graph-captured: every scene equals its standalone
+    render bit for bit (device code is compiled without FP contraction, so
+    the one-scene and many-scene kernels round alike); three scenes against
+    the oracle with common random numbers.
+c3  the gradient-check configuration (c0 scene at 128x128 with its 256^2
+    ground texture, 32 spp, depth 4, 65 materials x 6 parameters + the
+    texture): the image against its 16,384-spp FP64 reference; the adjoint
+    against the FP64 oracle adjoint; 16 seeded parameters against central
+    differences (h = 1e-3) with common random numbers -- of the GPU renderer
+    for albedo, specular and texels, of the oracle's detached estimator
+    (sampling held, BRDF values moved) for roughness (including 0.05) and
+    metallic.  Pass at 1e-2 relative.
+c4  one full-configuration multi-view gradient (the c1 geometry seen from 8
+    views at 512x512, 16 spp, depth 6, all 16 materials and texture 0)
+    against central differences of the summed loss on 6 parameters.
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import eleven
+import imagestats
+import oracle_py as orc
+from eleven import scenegen
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+C1_WINDOWS = [(448, 512, 180, 244), (928, 992, 508, 572), (1472, 1536, 760, 824)]
+
+
+def _crop(img, win, H):
+    x0, x1, y0, y1 = win
+    return img[H - y1:H - y0, x0:x1]
+
+
+@pytest.fixture(scope="module")
+def c0_scene():
+    return eleven.SceneData.load(scenegen.cached("sphere_field", n_spheres=64, res=(256, 256), seed=0))
+
+
+@pytest.fixture(scope="module")
+def c3_scene():
+    return eleven.SceneData.load(scenegen.cached("sphere_field", n_spheres=64, res=(128, 128), seed=0,
+                                                 ground_tex_size=256))
+
+
+@pytest.fixture(scope="module")
+def c1_scene():
+    print("[workloads] generating the 1,048,576-triangle heightfield scene", flush=True)
+    return eleven.SceneData.load(scenegen.cached("heightfield", res=(1920, 1080)))
+
+
+@pytest.fixture(scope="module")
+def c1_refbvh(c1_scene):
+    return orc.RefBVH(c1_scene.vertices())
+
+
+# ------------------------------------------------------------------ images --
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [eleven.MODE_WAVEFRONT, eleven.MODE_PLAIN], ids=["wavefront", "plain"])
+def test_c0_full_frame_matches_16k_reference(gpu_ctx, c0_scene, mode):
+    print(f"[workloads] c0 full frame vs 16k-spp FP64 reference, mode {mode}", flush=True)
+    ref = np.load(GOLDEN / "ref_c0_sphere_field_256.npz")
+    gpu_ctx.upload(c0_scene)
+    gpu_ctx.render(1024, depth=5, seed=70 + mode, mode=mode)
+    img, cnt, _, _ = gpu_ctx.read()
+    st = imagestats.compare(img[..., :3], cnt, ref["mean"], ref["var"].astype(np.float64), ref["count"])
+    imagestats.assert_consistent(st, f"c0 mode {mode}")
+
+
+@pytest.mark.gpu
+def test_c3_full_frame_matches_16k_reference(gpu_ctx, c3_scene):
+    print("[workloads] c3 frame vs 16k-spp FP64 reference", flush=True)
+    ref = np.load(GOLDEN / "ref_c3_sphere_field_128.npz")
+    gpu_ctx.upload(c3_scene)
+    gpu_ctx.render(2048, depth=4, seed=73)
+    img, cnt, _, _ = gpu_ctx.read()
+    st = imagestats.compare(img[..., :3], cnt, ref["mean"], ref["var"].astype(np.float64), ref["count"])
+    imagestats.assert_consistent(st, "c3")
+
+
+@pytest.mark.gpu
+def test_c1_crops_match_16k_reference(gpu_ctx, c1_scene):
+    print("[workloads] c1 crops vs 16k-spp FP64 reference", flush=True)
+    ref = np.load(GOLDEN / "ref_c1_heightfield_crops.npz")
+    assert [tuple(w) for w in ref["windows"]] == C1_WINDOWS
+    gpu_ctx.upload(c1_scene)
+    H = c1_scene.camera.y_res
+    gpu_ctx.render(1024, depth=8, seed=74)
+    img, cnt, _, _ = gpu_ctx.read()
+    g = np.stack([_crop(img[..., :3], w, H) for w in C1_WINDOWS])
+    n = np.stack([_crop(cnt, w, H) for w in C1_WINDOWS])
+    st = imagestats.compare(g, n, ref["mean"], ref["var"].astype(np.float64), ref["count"])
+    imagestats.assert_consistent(st, "c1 crops")
+
+
+@pytest.mark.gpu
+def test_c1_crops_match_oracle_common_random_numbers(gpu_ctx, c1_scene, c1_refbvh):
+    """The oracle replays the GPU's Philox streams on the three crops (FP32,
+    the build's point-light rule): accepted counts equal on almost every
+    pixel and almost every pixel within 1e-3 (paths diverge only where FP32
+    transcendental results differ in the last bit)."""
+    print("[workloads] c1 crops vs oracle, common random numbers", flush=True)
+    gpu_ctx.upload(c1_scene)
+    H = c1_scene.camera.y_res
+    spp, seed = 4, 5
+    gpu_ctx.render(spp, depth=8, seed=seed, flags=eleven.FLAG_NO_CULL)
+    img, cnt, _, _ = gpu_ctx.read()
+    for w in C1_WINDOWS:
+        s, _, c, _, _ = orc.render(c1_scene, c1_refbvh, spp, depth=8, seed=seed, precision=0, shadow_mode=1,
+                                   window=w)
+        om = _crop(s / np.maximum(c, 1)[..., None], w, H)
+        oc = _crop(c, w, H)
+        g = _crop(img[..., :3].astype(np.float64), w, H)
+        gc = _crop(cnt, w, H)
+        rel = np.abs(g - om) / (np.abs(om) + 1e-3)
+        frac = np.mean(rel.max(-1) < 1e-3)
+        print(f"  window {w}: counts equal {np.mean(gc == oc):.4f}, pixels within 1e-3 {frac:.4f}, "
+              f"mean rel {rel.mean():.2e}", flush=True)
+        assert np.mean(gc == oc) > 0.99
+        assert frac > 0.97 and rel.mean() < 2e-3
+
+
+# ------------------------------------------------------------------- batch --
+@pytest.fixture(scope="module")
+def c2_work():
+    print("[workloads] building the 32-scene batch workload", flush=True)
+    return scenegen.batch_workload(32, seed=0)
+
+
+@pytest.mark.gpu
+def test_c2_every_scene_of_the_batch_equals_its_standalone_render(gpu_ctx, c2_work):
+    print("[workloads] c2: 32 scenes, graph-captured batch vs standalone renders", flush=True)
+    b = eleven.Batch(0)
+    for s, spp, seed in c2_work:
+        b.add(s, spp, seed)
+    b.render(depth=6, graph=True)          # capture
+    st = b.render(depth=6, graph=True)     # replay
+    total = 0
+    for i, (s, spp, seed) in enumerate(c2_work):
+        img, cnt = b.read(i)
+        gpu_ctx.upload(s)
+        gpu_ctx.render(spp, depth=6, seed=seed, rng_scene=i)
+        ref, rcnt, rays, _ = gpu_ctx.read()
+        total += gpu_ctx.stats().rays
+        diff = int((cnt != rcnt).sum())
+        assert diff == 0, f"scene {i}: accepted counts differ on {diff} pixels"
+        assert np.array_equal(img, ref), f"scene {i}: {(img != ref).any(-1).sum()} pixels differ"
+    assert st.rays == total
+    print(f"  {st.rays} rays in {st.render_ms:.1f} ms", flush=True)
+    b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("i", [0, 13, 27])
+def test_c2_scenes_match_oracle_common_random_numbers(gpu_ctx, c2_work, i):
+    s, spp, seed = c2_work[i]
+    W, H = s.camera.x_res, s.camera.y_res
+    win = (W // 2 - 16, W // 2 + 16, H // 2 - 16, H // 2 + 16)
+    print(f"[workloads] c2 scene {i} ({s.tri_count} tris, {W}x{H}) vs oracle on {win}", flush=True)
+    gpu_ctx.upload(s)
+    gpu_ctx.render(4, depth=6, seed=seed, flags=eleven.FLAG_NO_CULL)
+    img, cnt, _, _ = gpu_ctx.read()
+    rb = orc.RefBVH(s.vertices())
+    o, _, oc, _, _ = orc.render(s, rb, 4, depth=6, seed=seed, precision=0, shadow_mode=1, window=win)
+    om = _crop(o / np.maximum(oc, 1)[..., None], win, H)
+    g = _crop(img[..., :3].astype(np.float64), win, H)
+    rel = np.abs(g - om) / (np.abs(om) + 1e-3)
+    assert np.mean(_crop(cnt, win, H) == _crop(oc, win, H)) > 0.99
+    assert np.mean(rel.max(-1) < 1e-3) > 0.97 and rel.mean() < 2e-3
+
+
+# ----------------------------------------------------------------- adjoint --
+def _perturbed(p0, rng):
+    pt = p0 + rng.normal(0, 0.1, p0.shape).astype(np.float32)
+    pt[:, :3] = np.clip(pt[:, :3], 0, 1)
+    pt[:, 3] = np.clip(pt[:, 3], 0.05, 1)
+    pt[:, 4:] = np.clip(pt[:, 4:], 0, 1)
+    return pt
+
+
+@pytest.mark.gpu
+def test_c3_adjoint_full_config(gpu_ctx, c3_scene):
+    from eleven.autograd import Renderer, material_params, with_params
+    print("[workloads] c3 adjoint: FP64 oracle adjoint and 16 central differences", flush=True)
+    scene = c3_scene
+    spp, depth, seed, h = 32, 4, 1, 1e-3
+    gpu_ctx.upload(scene)
+    r = Renderer(gpu_ctx, scene, spp=spp, depth=depth, seed=seed, grad_texture=0)
+    p0 = material_params(scene)
+    t0 = scene.texture(0).copy()
+    # the sphere whose albedo the image constrains most gets the lowest
+    # roughness c3 draws, 0.05, and is among the checked parameters
+    img = r.render(p0, t0)
+    g0, _ = gpu_ctx.adjoint(np.ones_like(img), grad_texture=0)
+    m_low = 1 + int(np.argmax(np.abs(g0[1:, :3]).sum(1)))
+    p0[m_low, 3] = 0.05
+    target = r.render(_perturbed(p0, np.random.default_rng(0)), np.clip(t0 * 1.2, 0, 1)).astype(np.float64)
+    img = r.render(p0, t0)
+    w = (2 * (img - target)).astype(np.float32)
+    dmat, dtex = gpu_ctx.adjoint(w, grad_texture=0)
+    # oracle adjoints on the same paths, the scene's materials set to p0
+    rb = orc.RefBVH(scene.vertices())
+    mats = (eleven.el_material * len(p0))(*with_params(scene.materials(), p0))
+    view = scene.view
+    saved = view.materials
+    view.materials = ctypes.cast(mats, ctypes.POINTER(eleven.el_material))
+    try:
+        o32 = orc.adjoint(scene, rb, w, spp, depth=depth, seed=seed, precision=0, shadow_mode=1, grad_texture=0)
+        o64 = orc.adjoint(scene, rb, w, spp, depth=depth, seed=seed, precision=1, shadow_mode=1, grad_texture=0)
+        # the FP32 oracle replays the same paths with the same texel indices
+        r32m = np.linalg.norm(dmat - o32[0]) / np.linalg.norm(o32[0])
+        r32t = np.linalg.norm(dtex - o32[1]) / np.linalg.norm(o32[1])
+        print(f"  vs FP32 oracle adjoint: materials {r32m:.2e}, texels {r32t:.2e}", flush=True)
+        assert r32m < 1e-3 and r32t < 1e-3
+        same = np.all(np.abs(o32[2] - o64[2]) <= 1e-3 * (np.abs(o64[2]) + 1e-3), axis=-1)
+        print(f"  pixels whose FP32 and FP64 paths agree: {same.mean():.4f}", flush=True)
+        # a pixel disagrees when any of its 32 paths branches differently in
+        # FP64 (~0.2 % of paths): about 7 % of the pixels
+        assert same.mean() > 0.9
+        wm = (w * same[..., None]).astype(np.float32)
+        gm, gt = gpu_ctx.adjoint(wm, grad_texture=0)
+        om, ot, _ = orc.adjoint(scene, rb, wm, spp, depth=depth, seed=seed, precision=1, shadow_mode=1,
+                                grad_texture=0)
+        rm = np.linalg.norm(gm - om) / np.linalg.norm(om)
+        big = np.abs(om) > 0.01 * np.abs(om).max()
+        worst = (np.abs(gm - om)[big] / np.abs(om)[big]).max()
+        # texel gradients: a hit within rounding of a texel edge reads the
+        # neighbouring texel in FP64 (nearest lookup), so single texels move;
+        # 4x4-texel block sums do not
+        blk = lambda t: t.reshape(64, 4, 64, 4, 3).sum((1, 3))
+        rt = np.linalg.norm(blk(gt) - blk(ot)) / np.linalg.norm(blk(ot))
+        print(f"  vs FP64 oracle adjoint: materials {rm:.2e}, worst parameter {worst:.2e}, "
+              f"texels (4x4 blocks) {rt:.2e}", flush=True)
+        assert rm < 1e-2 and rt < 1e-2 and worst < 1e-2
+        # 16 parameters vs central differences
+        sel = np.random.default_rng(2)
+        big_g = np.abs(dmat) > 1e-3 * np.abs(dmat).max()
+        plain = [(m, k) for m in range(len(p0)) for k in (0, 1, 2, 5)
+                 if big_g[m, k] and not (k < 3 and scene.materials()[m].albedo_tex >= 0)]
+        # metallic is 0 or 1 in c3 (Bernoulli): at 0 the derivative of m^2.2 is
+        # exactly 0 and m - h has no power; 1 +- h is fine
+        detached = [(m, k) for m in range(len(p0)) for k in (3, 4)
+                    if big_g[m, k] and p0[m, k] > h and (m, k) != (m_low, 3)]
+        kinds = {k for _, k in detached}
+        assert kinds == {3, 4}, kinds
+        picks = [plain[i] for i in sel.choice(len(plain), 6, replace=False)]
+        dpicks = [(m_low, 3)] + [detached[i] for i in sel.choice(len(detached), 5, replace=False)]
+        tex_idx = np.flatnonzero(np.abs(dtex) > 1e-2 * np.abs(dtex).max())
+        tpicks = [np.unravel_index(i, dtex.shape) for i in sel.choice(tex_idx, 16 - len(picks) - len(dpicks),
+                                                                      replace=False)]
+        errs = []
+        loss = lambda im: float(((im.astype(np.float64) - target) ** 2).sum())
+        for (m, k) in picks:                 # the GPU renderer itself (sampling does not move)
+            pp, pm = p0.copy(), p0.copy()
+            pp[m, k] += h
+            pm[m, k] -= h
+            fd = (loss(r.render(pp, t0)) - loss(r.render(pm, t0))) / float(pp[m, k] - pm[m, k])
+            errs.append((f"material {m} param {k}", fd, dmat[m, k]))
+        for (y, x, c) in tpicks:
+            tp, tm = t0.copy(), t0.copy()
+            tp[y, x, c] += h
+            tm[y, x, c] -= h
+            fd = (loss(r.render(p0, tp)) - loss(r.render(p0, tm))) / float(tp[y, x, c] - tm[y, x, c])
+            errs.append((f"texel ({y},{x},{c})", fd, dtex[y, x, c]))
+        # roughness / metallic: the detached estimator of the oracle (FP64), on
+        # the pixels whose paths agree, against the adjoint of the same weights
+        for (m, k) in dpicks:
+            vals = []
+            for sgn in (1, -1):
+                pe = p0.copy()
+                pe[m, k] += sgn * h
+                em = (eleven.el_material * len(pe))(*with_params(scene.materials(), pe))
+                s, _, c, _, _ = orc.render(scene, rb, spp, depth=depth, seed=seed, precision=1, shadow_mode=1,
+                                           eval_materials=em)
+                vals.append(float((wm.astype(np.float64) * (s / np.maximum(c, 1)[..., None])).sum()))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            errs.append((f"material {m} param {k} (detached)", fd, gm[m, k]))
+    finally:
+        view.materials = saved
+    worst = 0.0
+    for name, fd, g in errs:
+        e = abs(fd - g) / max(abs(fd), 1e-12)
+        worst = max(worst, e)
+        print(f"  {name}: fd {fd:.6e} adjoint {g:.6e} rel {e:.2e}", flush=True)
+    assert len(errs) == 16
+    assert worst < 1e-2, worst
+
+
+@pytest.mark.gpu
+def test_c4_multiview_gradient_matches_central_differences(gpu_ctx, c1_scene):
+    from eleven import inverse
+    print("[workloads] c4: 8-view gradient at 512x512 vs central differences", flush=True)
+    scene = c1_scene
+    gpu_ctx.upload(scene)
+    cams = inverse.orbit_views(scene.camera, 8, x_span=16.0, res=(512, 512))
+    mv = inverse.MultiView(gpu_ctx, scene, cams, spp=16, depth=6, grad_texture=0)
+    truth = inverse.material_params(scene)
+    t_true = scene.texture(0).copy()
+    targets = [im.astype(np.float64) for im in mv.images(truth, t_true, seed=777, spp=64)]
+    p0 = _perturbed(truth, np.random.default_rng(3))
+    t0 = np.clip(t_true * 0.8 + 0.1, 0, 1).astype(np.float32)
+    seed, h = 11, 1e-3
+    _, dmat, dtex = mv.loss_and_grad(p0, t0, targets, seed=seed, decorrelate=False)
+
+    def loss(p, t):
+        return sum(float(((im.astype(np.float64) - tg) ** 2).sum())
+                   for im, tg in zip(mv.images(p, t, seed=seed), targets))
+
+    order = np.argsort(-np.abs(dmat[:, 5]))
+    picks = [(int(m), 5) for m in order[:3]]                     # specular of the most visible materials
+    tex = np.argsort(-np.abs(dtex).ravel())[:3]
+    tpicks = [np.unravel_index(i, dtex.shape) for i in tex]
+    worst = 0.0
+    for (m, k) in picks:
+        pp, pm = p0.copy(), p0.copy()
+        pp[m, k] += h
+        pm[m, k] -= h
+        fd = (loss(pp, t0) - loss(pm, t0)) / float(pp[m, k] - pm[m, k])
+        e = abs(fd - dmat[m, k]) / max(abs(fd), 1e-12)
+        print(f"  material {m} specular: fd {fd:.6e} adjoint {dmat[m, k]:.6e} rel {e:.2e}", flush=True)
+        worst = max(worst, e)
+    for (y, x, c) in tpicks:
+        tp, tm = t0.copy(), t0.copy()
+        tp[y, x, c] += h
+        tm[y, x, c] -= h
+        fd = (loss(p0, tp) - loss(p0, tm)) / float(tp[y, x, c] - tm[y, x, c])
+        e = abs(fd - dtex[y, x, c]) / max(abs(fd), 1e-12)
+        print(f"  texel ({y},{x},{c}): fd {fd:.6e} adjoint {dtex[y, x, c]:.6e} rel {e:.2e}", flush=True)
+        worst = max(worst, e)
+    assert worst < 1e-2, worst

@@ -209,12 +209,87 @@ def test_hdri_lookup_equals_reference_search(which):
     assert np.array_equal(eleven.hdri_lookup(rgb, tie), [orc.hdri_search(cdf, r, L) for r in tie])
 
 
+def test_hdri_lookup_exhaustive_on_the_workload_sky(tmp_path):
+    """The 2048x1024 sky of the heightfield workload (gradient + a 2-degree
+    sun lobe of peak 1e3, RGBE-quantised through the .hdr loader, so many
+    pixels share values and the CDF has long runs of ties): the renderer's
+    guide-table lookup (2^21 cells) returns binarySearch(cdf, r)
+    (HDRI.hpp:130-142) for every one of the 2^24 values r the renderer
+    draws.  The search's answer is constant for r strictly between two
+    consecutive distinct CDF values (every comparison it makes is against a
+    CDF value), so the reference answer of every r comes from one search per
+    such interval plus one per r equal to a CDF value."""
+    import estimator_expectation as ee
+    d = tmp_path / "sky_scene"
+    d.mkdir()
+    w = scenegen.ObjWriter()
+    w.add_mesh("t", "m", np.array([(0, 0, 0), (1, 0, 0), (0, 1, 0)], float), np.zeros((3, 2)),
+               np.tile([0.0, 0.0, 1.0], (3, 1)), np.array([(0, 1, 2)]))
+    w.write(d / "scene.obj")
+    scenegen.write_mtl(d / "scene.mtl", [{"name": "m"}])
+    (d / "HDRI").mkdir()
+    scenegen.write_hdr(d / "HDRI" / "sky.hdr", scenegen.sky_hdr(2048, 1024))
+    scenegen.write_json(d / "scene.json", {"xRes": 8, "yRes": 8}, {"name": "sky"}, [])
+    scene = eleven.SceneData.load(d)
+    rgb = scene.hdri().copy()             # the view lives as long as the scene
+    assert rgb.shape == (1024, 2048, 3)
+    cdf, _ = orc.hdri_cdf(rgb)
+    L = rgb.shape[0] * rgb.shape[1]
+    c = np.unique(cdf[:L].astype(np.float64))
+    i = np.arange(1 << 24, dtype=np.int64)
+    r = i / float(1 << 24)
+    j = np.searchsorted(c, r)                          # c[j-1] < r <= c[j]
+    tie = (j < c.size) & (c[np.minimum(j, c.size - 1)] == r)
+    # one representative grid value per interval (c[j-1], c[j]): the first one inside
+    first = np.full(c.size + 1, -1, np.int64)
+    inner = ~tie
+    jj, pos = np.unique(j[inner], return_index=True)
+    first[jj] = i[inner][pos]
+    ans_int = np.full(c.size + 1, -1, np.int64)
+    ans_int[jj] = ee.binary_search(cdf, (first[jj] / float(1 << 24)).astype(np.float32), L)
+    want = ans_int[j]
+    want[tie] = ee.binary_search(cdf, r[tie].astype(np.float32), L)
+    got = eleven.hdri_lookup(rgb, r.astype(np.float32))
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, f"{bad.size} mismatches, first r = {r[bad[:5]]}"
+    assert tie.sum() > 0 and np.unique(want).size > 0.5 * L
+
+
 def test_hdri_lookup_rejects_out_of_range():
     with pytest.raises(eleven.ElevenError):
         eleven.hdri_lookup(np.ones((2, 2, 3), np.float32), np.array([1.0], np.float32))
 
 
 # --------------------------------------------------------------- imageio --
+def test_fast_pow_pinned_by_hand_computed_values(tmp_path):
+    """Ankerl's approximation as the reference's stb_image carries it
+    (stb_image.h:127-136): the high word of the double becomes
+    (int)(b * (hi - 1072632447) + 1072632447), the low word 0.  Worked by hand:
+    fastPow(2, 0.5): hi 0x40000000 -> (int)(0.5 * 1109377 + 1072632447) =
+    1073187135 = 0x3FF7893F, i.e. 1 + 0x7893F / 2^20; fastPow(0.5, 2.2): hi
+    0x3FE00000 -> 0x3FCDE9CE; fastPow(1, 1/2.2) -> 0x3FEF7E73 (not 1: the
+    constant is not 1.0's high word); fastPow(0.25, 1/2.2) -> 0x3FE0F2D0.
+    The loader's byte -> value table uses gamma 2.2f (a float,
+    stb_image.h:1863): byte 255 -> 0x3FF11D01 (1.0696, not 1), byte 1 ->
+    0x3ED7A6BD (0x3ED7A6BE with a double 2.2)."""
+    cases = [(2.0, 0.5, "0x1.7893fp+0"), (0.5, 2.2, "0x1.de9cep-3"), (1.0, 1 / 2.2, "0x1.f7e73p-1"),
+             (0.25, 1 / 2.2, "0x1.0f2dp-1")]
+    for a, b, h in cases:
+        assert imageio.fast_pow(np.array([a]), b)[0] == float.fromhex(h), (a, b)
+    # the CLI's output mapping (main.cpp:157): (uint8)(fastPow(0.25, 1/2.2) * 255) = 135
+    assert imageio.to_ldr(np.full((1, 1, 4), 0.25, np.float32))[0, 0, 0] == 135
+    # the loader's table, read back from an sRGB albedo map
+    d = tmp_path / "fp"
+    d.mkdir()
+    grey = np.array([1, 64, 128, 200, 255], np.uint8)
+    scenegen.write_bmp(d / "g.bmp", np.repeat(grey[None, :, None], 3, -1))
+    _write_scene(d, "o a\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n", "newmtl m\nmap_Kd g.bmp\n")
+    t = eleven.SceneData.load(d).texture(0)[0, :, 0].copy()
+    want = [float.fromhex(h) for h in ("0x1.7a6bdp-18", "0x1.ad9f1p-5", "0x1.e0d24p-3", "0x1.1edd1p-1",
+                                       "0x1.11d01p+0")]
+    assert t.tolist() == want
+
+
 def test_fast_pow_bit_trick_properties():
     a = np.array([1.0, 0.25, 0.5, 0.75])
     assert imageio.fast_pow(a[:1], 1.0)[0] == 1.0                   # low word of 1.0 is zero

@@ -757,7 +757,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     c->stats.shade_ms = shadeMs;
     c->stats.shadow_ms = shadowMs;
     c->stats.extend_launches = extendLaunches;
-    c->stats.extend_rays = 0;
+    c->stats.extend_rays = tally(h, kStatExtRays);
     c->stats.node_fetches = tally(h, kStatNodes);
     c->stats.tri_tests = tally(h, kStatTris);
     if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack or queue overflow (results incomplete)");

@@ -94,6 +94,7 @@ __device__ __forceinline__ WorkSource chunkSource(uint32_t* qcounters, int kind,
     static_assert(kChunkShares == (int)kWorkShares, "one chunk share per lane");
     ws.parts = WIDE ? kChunkShares : kXcds;
     ws.home = blockIdx.x % ws.parts;
+    ws.fixed = ws.chunks <= waves;
     return ws;
 }
 // Zeroes the chunk counters of `kind` (threads 0..kChunkShares-1 of block 0).
@@ -838,9 +839,9 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
         q.hits[qm.entry(i, segSize)] = make_float4(__int_as_float(h.tri), h.t, h.u, h.v);
     };
     traceWaveRange<W, false, COUNT, P::kMany>(s0, st, ws, refill, load, store, cnt, rays);
-    const int slots[4] = {kStatRays, kStatOverflow, kStatNodes, kStatTris};
-    const uint32_t vals[4] = {rays, (uint32_t)st.overflow, cnt.nodes, cnt.tris};
-    blockFlush<4>(counters, slots, vals);
+    const int slots[5] = {kStatRays, kStatOverflow, kStatNodes, kStatTris, kStatExtRays};
+    const uint32_t vals[5] = {rays, (uint32_t)st.overflow, cnt.nodes, cnt.tris, rays};
+    blockFlush<5>(counters, slots, vals);
 }
 
 // Adds `c` to a path's radiance.  The first bounce writes it (the radiance

@@ -27,7 +27,8 @@ constexpr int kAdjMaxDepthHost = 16;   // deepest path the adjoint records (k_ad
 
 // Tally slots (per XCD copy): the host sums the kXcds copies.
 constexpr int kStatSlots = 8;
-constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5;
+constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5,
+              kStatExtRays = 6;   // rays of the closest-hit (extension) launches
 
 // Queue counters: kXcds per queue.
 constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadowH = 2 * kXcds, kQShadowP = 3 * kXcds;

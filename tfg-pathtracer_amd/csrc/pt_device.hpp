@@ -963,18 +963,27 @@ struct BvhPtr {
 // XCD draw neighbouring chunks, which keeps their rays coherent in its L2),
 // a batch 64 (measured faster there).  Every counter starts the launch at 0.  A share is skipped without an atomic once a coherent read shows it
 // drawn out (a read never runs ahead of the counter, so no chunk is lost).
+// A queue with no more chunks than the grid has waves (the late bounces, small
+// frames) is handed out without counters: wave w takes chunk w -- the chunk
+// the dynamic scheme's share order would also give its XCD -- and waves past
+// the last chunk return at once.  Thousands of waves contending for eight
+// counters cost such a launch ~100 us whatever its size.
 constexpr uint32_t kWorkShares = 64;   // shares of a wide WorkSource (one per lane of a wave)
 struct WorkSource {
     uint32_t begin, end;
     uint32_t* ctr;
     uint32_t n, chunk, stride, parts, home;
     uint32_t chunks;   // ceil(n / chunk)
+    bool fixed;        // chunks <= waves of the grid: wave w takes chunk w, no counter atomics
 };
 
 template <int W, bool ANY, bool COUNT, bool PER_RAY_BVH = false, class Load, class Store>
 __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st, const WorkSource& ws,
                                                uint32_t refill, Load&& load, Store&& store, TraceCounters& cnt,
                                                uint32_t& traced) {
+    const uint32_t waveId = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ws.ctr && ws.fixed && waveId >= ws.chunks) return;   // no chunk for this wave
+    bool fixedDrawn = false;
     BvhPtr bvh{(const uint4*)s.nodes, s.tris};
     // One BVH for the whole wave: its root node (visited first by every ray)
     // is read once into scalar registers, and a new ray is tested against it
@@ -1007,12 +1016,20 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
         const bool want = nIdle >= refill || nIdle == 64;
         if (want && cursor >= end) {
             more = false;
+            if (ws.ctr && ws.fixed) {   // the wave's one chunk
+                if (!fixedDrawn) {
+                    fixedDrawn = true;
+                    cursor = waveId * ws.chunk;
+                    end = cursor + ws.chunk < ws.n ? cursor + ws.chunk : ws.n;
+                    more = cursor < end;
+                }
+            }
             // Wide sources (batches, parts == 64): lane l looks at share
             // (home + l) % 64 at once (one load per lane instead of 64
             // dependent loads; the whole wave runs here), and the wave draws
             // from the first share with chunks left.  Share x holds the
             // chunks x, x + 64, ...: (chunks - x + 63) / 64 of them.
-            while (PER_RAY_BVH && ws.ctr) {
+            while (PER_RAY_BVH && ws.ctr && !ws.fixed) {
                 const uint32_t x = (ws.home + lane) & (kWorkShares - 1);
                 const uint32_t seen =
                     __hip_atomic_load(ws.ctr + x * ws.stride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1034,7 +1051,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
             // Narrow sources (one scene, parts == 8): one share at a time,
             // from the wave's home share on; a share found drawn out is not
             // looked at again.
-            while (!PER_RAY_BVH && ws.ctr && tried < ws.parts) {
+            while (!PER_RAY_BVH && ws.ctr && !ws.fixed && tried < ws.parts) {
                 const uint32_t x = (ws.home + tried) % ws.parts;
                 uint32_t* const c = ws.ctr + x * ws.stride;
                 const uint32_t seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

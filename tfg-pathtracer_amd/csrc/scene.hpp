@@ -103,6 +103,10 @@ struct Tri {
     int objectID = 0;
 };
 
+// Mikkelsen tangent frames of one mesh object's triangles (tangent_space.cpp):
+// sets every corner's tangent and each triangle's tangentsSign.
+void mikkTangents(std::vector<Tri>& tris);
+
 struct MeshObject {
     std::string name;
     std::string matName;

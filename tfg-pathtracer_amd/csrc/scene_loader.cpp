@@ -11,9 +11,10 @@
 //   numbers may use exponents; faces with more than 3 vertices are fanned;
 //   negative (relative) OBJ indices are resolved; MTL keys `Pr` / `Pm` set
 //   scalar roughness / metallic (the reference has no scalar keys for them).
-// Tangents: per-triangle UV tangents orthogonalised against each corner
-// normal (Lengyel 2001) instead of MikkTSpace; they only feed normal maps
-// and the TANGENT/BITANGENT passes.
+// Tangents: Mikkelsen's tangent space per mesh object (tangent_space.cpp),
+// the reference's MikkTSpace convention (ObjLoader.hpp:167-168,
+// mikktspaceCallback.hpp:27-99); they feed normal maps and the
+// TANGENT/BITANGENT passes.
 #include "scene.hpp"
 
 #include <algorithm>
@@ -543,40 +544,6 @@ static std::vector<UnloadedMaterial> loadMtls(const std::string& path) {
     return out;
 }
 
-// Tangent frame per triangle corner (see header comment).
-static void computeTangents(std::vector<Tri>& tris) {
-    for (Tri& t : tris) {
-        Vector3 e1(t.vertices[1].x - t.vertices[0].x, t.vertices[1].y - t.vertices[0].y,
-                   t.vertices[1].z - t.vertices[0].z);
-        Vector3 e2(t.vertices[2].x - t.vertices[0].x, t.vertices[2].y - t.vertices[0].y,
-                   t.vertices[2].z - t.vertices[0].z);
-        float du1 = t.uv[1].x - t.uv[0].x, dv1 = t.uv[1].y - t.uv[0].y;
-        float du2 = t.uv[2].x - t.uv[0].x, dv2 = t.uv[2].y - t.uv[0].y;
-        float det = du1 * dv2 - du2 * dv1;
-        Vector3 T, B;
-        if (std::fabs(det) > 1e-20f) {
-            float r = 1.0f / det;
-            T = Vector3((e1.x * dv2 - e2.x * dv1) * r, (e1.y * dv2 - e2.y * dv1) * r, (e1.z * dv2 - e2.z * dv1) * r);
-            B = Vector3((e2.x * du1 - e1.x * du2) * r, (e2.y * du1 - e1.y * du2) * r, (e2.z * du1 - e1.z * du2) * r);
-        } else {
-            T = normalizedRef(e1);
-            Vector3 n(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
-            B = Vector3(n.y * T.z - n.z * T.y, n.z * T.x - n.x * T.z, n.x * T.y - n.y * T.x);
-        }
-        float signSum = 0;
-        for (int i = 0; i < 3; i++) {
-            const Vector3& n = t.normals[i];
-            float nd = n.x * T.x + n.y * T.y + n.z * T.z;
-            Vector3 to(T.x - n.x * nd, T.y - n.y * nd, T.z - n.z * nd);
-            to = normalizedRef(to);
-            if (to.x == 0 && to.y == 0 && to.z == 0) to = normalizedRef(T);
-            t.tangents[i] = to;
-            Vector3 c(n.y * to.z - n.z * to.y, n.z * to.x - n.x * to.z, n.x * to.y - n.y * to.x);
-            signSum += (c.x * B.x + c.y * B.y + c.z * B.z) < 0 ? -1.f : 1.f;
-        }
-        t.tangentsSign = signSum < 0 ? -1.f : 1.f;
-    }
-}
 
 static long parseIndex(const char* s, size_t n, long count) {
     // 1-based; negative = relative to the end (superset of the reference).
@@ -668,7 +635,7 @@ static void loadObjs(const std::string& path, std::vector<MeshObject>& meshes,
             }
         }
     }
-    for (auto& t : meshTris) computeTangents(t);
+    for (auto& t : meshTris) mikkTangents(t);   // per mesh object, as ObjLoader.hpp:167-168
 }
 
 void Scene::addMeshObject(MeshObject mo, const std::vector<Tri>& meshTris) {

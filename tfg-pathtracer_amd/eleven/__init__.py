@@ -251,23 +251,34 @@ class SceneData:
             self._handle = None
 
     # ---- accessors (numpy views of the flat arrays) -------------------------
+    # The views point into memory this SceneData owns (the loader's or the
+    # arrays from_arrays keeps); each view holds a reference to the SceneData,
+    # so the memory lives as long as any view of it.
+    def _view(self, ptr, shape, ctype=C.c_float, dtype=np.float32):
+        n = int(np.prod(shape))
+        if n == 0 or not ptr:
+            return np.zeros(shape, dtype)
+        buf = (ctype * n).from_address(C.addressof(ptr.contents))
+        buf._owner = self
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
     @property
     def tri_count(self) -> int:
         return self.view.tri_count
 
     def vertices(self) -> np.ndarray:
-        return np.ctypeslib.as_array(self.view.vertices, (self.view.tri_count, 9))
+        return self._view(self.view.vertices, (self.view.tri_count, 9))
 
     def materials(self) -> list:
         return [self.view.materials[i] for i in range(self.view.material_count)]
 
     def texture(self, i: int) -> np.ndarray:
         t = self.view.textures[i]
-        return np.ctypeslib.as_array(t.data, (t.height, t.width, 3))
+        return self._view(t.data, (t.height, t.width, 3))
 
     def hdri(self) -> np.ndarray:
         t = self.view.hdri
-        return np.ctypeslib.as_array(t.data, (t.height, t.width, 3))
+        return self._view(t.data, (t.height, t.width, 3))
 
     def hdri_cdf(self):
         """(cdf[w*h+1], radianceSum) as built by HDRI::generateCDF; loaded scenes only."""
