@@ -252,17 +252,16 @@ int  el_set_camera(el_ctx* ctx, const el_camera* camera);
  * el_read_passes.  Each path is replayed from its RNG counters, nothing of a
  * path outlives the thread that replays it; a material whose albedo comes
  * from a texture gets no albedo gradient (its texels do, if selected).
- * Without a recording render
- * each path is walked twice: once traced, for its radiance, keeping the
- * closest hits of its vertices in the walking thread's own slots (a buffer of
- * resident threads x depth x 16 B, independent of the pixel and sample
- * counts), then replayed from those without tracing.  With `device_ptrs` != 0
- * the three arrays are device pointers.  d_materials and d_texture may be NULL:
+ * Without a recording render each path is walked twice: once traced, for its
+ * radiance, keeping the closest hits of its vertices in the walking thread's
+ * own slots (a buffer of resident threads x depth x 16 B, independent of the
+ * pixel and sample counts), then replayed from those without tracing.  If
+ * the render used EL_FLAG_RECORD_PATHS, each path is walked once (the
+ * recorded radiance gives the radiance after each vertex), from its recorded
+ * vertices when the render kept them.  With `device_ptrs` != 0 the three
+ * arrays are device pointers.  d_materials and d_texture may be NULL:
  * nothing is copied out, and the sums stay in the context for following
- * EL_ADJ_ACCUMULATE calls.  If that render used
- * EL_FLAG_RECORD_PATHS, each path is walked once (the recorded radiance
- * gives the radiance after each vertex), from its recorded vertices when the
- * render kept them. */
+ * EL_ADJ_ACCUMULATE calls. */
 int  el_adjoint(el_ctx* ctx, const el_adjoint_params* params, const float* d_image,
                 float* d_materials, float* d_texture, int device_ptrs);
 

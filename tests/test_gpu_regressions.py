@@ -13,6 +13,11 @@
    that asks for it afterwards fails instead of using the previous view's.
 3. A batch rendered with a captured graph and then with EL_BATCH_NO_CULL
    replays the faithful launch sequence, not the culling one.
+4. The wavefront renderer's per-path AOV buffer grows with the generation:
+   a context whose other wavefront buffers a large non-AOV render has grown,
+   and whose AOV buffer a small AOV render has sized, renders a larger AOV
+   frame exactly as a fresh context does (the buffer was once sized by the
+   first AOV render only, and a larger one then wrote past its end).
 """
 import numpy as np
 import pytest
@@ -104,3 +109,35 @@ def test_batch_graph_keyed_on_culling(gpu_ctx):
         assert b.render(depth=5, graph=True).rays == eager_cull
         assert b.render(depth=5, graph=True, cull=False).rays == eager_faithful
     b.close()
+
+
+@pytest.mark.gpu
+def test_aov_buffer_grows_with_the_generation():
+    print("[regress] wavefront AOV buffer: small AOV render, then a larger one", flush=True)
+    small = scenegen.sphere_field_arrays(n_spheres=6, res=(32, 32), seed=5)
+    large = scenegen.sphere_field_arrays(n_spheres=6, res=(48, 40), seed=6)
+    used = eleven.Context(0)
+    try:
+        used.upload(small)
+        used.render(64, depth=3, seed=2)                      # 65,536 paths: wavefront buffers grow
+        used.clear()
+        used.render(2, depth=3, seed=2, flags=eleven.FLAG_AOV)   # 2,048 paths: sizes the AOV buffer
+        used.upload(large)
+        st = used.render(8, depth=3, seed=3, flags=eleven.FLAG_AOV)   # 15,360 paths
+        got = [used.read_aov(p) for p in (eleven.PASS_NORMAL, eleven.PASS_TANGENT, eleven.PASS_BITANGENT)]
+        got_img = used.read()[0]
+    finally:
+        used.close()
+    fresh = eleven.Context(0)
+    try:
+        fresh.upload(large)
+        fresh.render(8, depth=3, seed=3, flags=eleven.FLAG_AOV)
+        ref = [fresh.read_aov(p) for p in (eleven.PASS_NORMAL, eleven.PASS_TANGENT, eleven.PASS_BITANGENT)]
+        ref_img = fresh.read()[0]
+    finally:
+        fresh.close()
+    assert st.paths == 48 * 40 * 8
+    assert np.array_equal(got_img, ref_img)
+    for g, r in zip(got, ref):
+        assert np.array_equal(g, r)
+    assert np.abs(ref[0]).max() > 0.1

@@ -94,7 +94,6 @@ struct el_ctx {
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
     DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters;
     size_t wfCap = 0, wfQcap = 0;
-    bool wfAov = false;
 
     DevBuf tRays, tTmax, tTri, tT, tUV, tOcc;   // scratch for host-pointer ray queries
 
@@ -516,9 +515,13 @@ static hipEvent_t poolEvent(el_ctx* c, size_t i) {
 }
 
 // Path slots: `cap`; queue entries: max(cap, kXcds * segSize) (the camera
-// queue is indexed by slot, the compacted queues by XCD segment).
+// queue is indexed by slot, the compacted queues by XCD segment).  Every
+// buffer keeps the largest size asked of it (DevBuf::ensure); the per-path
+// AOV buffer is sized by this call's capacity whatever the others hold, since
+// renders without AOV may have grown them past its size.
 static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
-    if (cap <= c->wfCap && qcap <= c->wfQcap && (!aov || c->wfAov)) return EL_OK;
+    if (aov) EL_HIP(c, c->wAov.ensure(cap * 48));
+    if (cap <= c->wfCap && qcap <= c->wfQcap) return EL_OK;
     EL_HIP(c, c->wRay0.ensure(qcap * 32));
     EL_HIP(c, c->wRay1.ensure(qcap * 32));
     EL_HIP(c, c->wHits.ensure(qcap * 16));
@@ -529,11 +532,9 @@ static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
     EL_HIP(c, c->wPix.ensure(cap * 4));
     EL_HIP(c, c->wSample.ensure(cap * 4));
     EL_HIP(c, c->wQuiet.ensure(cap * 8));
-    if (aov) EL_HIP(c, c->wAov.ensure(cap * 48));
     EL_HIP(c, c->wCounters.ensure(kQCounters * sizeof(uint32_t)));
     c->wfCap = std::max(c->wfCap, cap);
     c->wfQcap = std::max(c->wfQcap, qcap);
-    c->wfAov = c->wfAov || aov;
     return EL_OK;
 }
 
@@ -566,6 +567,11 @@ static int wfPlan(el_ctx* c, const LaunchCfg& lc, uint64_t total, uint64_t npad,
     const uint64_t qcap = std::max<uint64_t>(Q, kXcds * segSize);
     int rc = ensureWf(c, Q, qcap, aov);
     if (rc) return rc;
+    // the launches index Q path slots and qcap queue entries in every buffer
+    if (c->wRay0.n < qcap * 32 || c->wRay1.n < qcap * 32 || c->wHits.n < qcap * 16 || c->wShH.n < qcap * 48 ||
+        c->wShP.n < qcap * 48 || c->wT.n < Q * 16 || c->wL.n < Q * 16 || c->wPix.n < Q * 4 || c->wSample.n < Q * 4 ||
+        c->wQuiet.n < Q * 8 || (aov && c->wAov.n < Q * 48))
+        return fail(c, EL_ERR_STATE, "wavefront buffers smaller than the generation (internal error)");
     q.ray[0] = c->wRay0.as<float4>();
     q.ray[1] = c->wRay1.as<float4>();
     q.hits = c->wHits.as<float4>();
