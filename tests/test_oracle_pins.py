@@ -154,6 +154,104 @@ def test_disney_eval_reciprocity_and_hemispheres():
         assert orc.disney_pdf(hd, -a, -b, 1) == 1.0               # reference returns 1 there
 
 
+def _mirror(theta):
+    """V and L mirrored about N = +z at angle theta in the xz-plane (H = N,
+    L.H = cos theta); returns the ray direction (-V) and L."""
+    c, s_ = math.cos(theta), math.sin(theta)
+    return np.array([s_, 0.0, -c]), np.array([s_, 0.0, c])
+
+
+@pytest.mark.parametrize("metallic", [0.0, 0.3])
+@pytest.mark.parametrize("tint", [0.0, 1.0])
+def test_disney_sheen_lobe_closed_form(metallic, tint):
+    """Sheen adds FH sheen Csheen (1 - m) (Disney.hpp:236): at theta = 60 deg
+    mirrored, L.H = 1/2 so FH = (1/2)^5 = 1/32; Csheen = 1 without tint and
+    albedo / (0.3 r + 0.6 g + 0.1 b) with full tint."""
+    alb = np.array([0.6, 0.3, 0.2])
+    d, L = _mirror(math.pi / 3)
+    base = orc.disney_eval(_hd(albedo=alb, metallic=metallic, sheen=0.0, sheenTint=tint), d, L, 1)
+    with_ = orc.disney_eval(_hd(albedo=alb, metallic=metallic, sheen=0.7, sheenTint=tint), d, L, 1)
+    csheen = alb / (0.3 * 0.6 + 0.6 * 0.3 + 0.1 * 0.2) if tint else np.ones(3)
+    np.testing.assert_allclose(with_ - base, 0.7 * csheen * (1 - metallic) / 32, rtol=1e-12)
+    # normal incidence: FH = 0, no sheen at all
+    n0 = orc.disney_eval(_hd(albedo=alb, metallic=metallic, sheen=0.7, sheenTint=tint), (0, 0, -1), (0, 0, 1), 1)
+    n1 = orc.disney_eval(_hd(albedo=alb, metallic=metallic, sheen=0.0, sheenTint=tint), (0, 0, -1), (0, 0, 1), 1)
+    np.testing.assert_allclose(n0, n1, rtol=1e-14)
+
+
+def test_disney_subsurface_lobe_closed_form():
+    """Subsurface replaces Fd by ss in the diffuse term (Disney.hpp:213-222,
+    238), so f(ss=1) - f(ss=0) = (ss - Fd) Cdlin (1 - m) / pi.  Hand-worked
+    at roughness 1/2: normal incidence (FL = FV = 0): Fd = 1, Fss = 1,
+    ss = 1.25 (1 (1/2 - 1/2) + 1/2) = 0.625; theta = 60 deg mirrored
+    (N.L = N.V = L.H = 1/2, FL = FV = 1/32): Fd90 = 0.75, Fd = (1 - 0.25/32)^2
+    = 0.98443603515625, Fss90 = 0.125, Fss = (1 - 0.875/32)^2 =
+    0.9460601806640625, ss = 1.25 (Fss (1 - 1/2) + 1/2) =
+    1.2162876129150390625."""
+    alb, m = np.array([0.8, 0.5, 0.3]), 0.25
+    for (d, L), expect in [(((0, 0, -1), (0, 0, 1)), 0.625 - 1.0),
+                           (_mirror(math.pi / 3), 1.2162876129150390625 - 0.98443603515625)]:
+        f1 = orc.disney_eval(_hd(albedo=alb, metallic=m, roughness=0.5, subsurface=1.0), d, L, 1)
+        f0 = orc.disney_eval(_hd(albedo=alb, metallic=m, roughness=0.5, subsurface=0.0), d, L, 1)
+        np.testing.assert_allclose((f1 - f0) * math.pi / (alb * (1 - m)), expect, rtol=1e-12)
+
+
+@pytest.mark.parametrize("gloss", [0.0, 0.5, 1.0])
+def test_disney_clearcoat_lobe_closed_form(gloss):
+    """Clearcoat adds 0.25 cc Gr Fr Dr (Disney.hpp:232-238): Dr = GTR1 with
+    a = 0.1 + (0.001 - 0.1) gloss, which at N.H = 1 is (a^2 - 1) / (pi
+    ln(a^2) a^2); Gr = G(N.L) G(N.V) with G(x) = 1 / (x + sqrt(1/16 + x^2 -
+    x^2 / 16)); Fr = 0.04 + 0.96 FH.  Normal incidence: Gr = 1/4, Fr = 0.04;
+    theta = 60 deg mirrored: FH = 1/32, Fr = 0.07."""
+    a = 0.1 + (0.001 - 0.1) * gloss
+    dr = (a * a - 1) / (math.pi * math.log(a * a) * a * a)
+    g = lambda x: 1.0 / (x + math.sqrt(0.0625 + x * x - 0.0625 * x * x))
+    for (d, L), gr, fr in [(((0, 0, -1), (0, 0, 1)), 0.25, 0.04), (_mirror(math.pi / 3), g(0.5) ** 2, 0.07)]:
+        f1 = orc.disney_eval(_hd(clearcoat=0.6, clearcoatGloss=gloss), d, L, 1)
+        f0 = orc.disney_eval(_hd(clearcoat=0.0, clearcoatGloss=gloss), d, L, 1)
+        # 1e-9: at gloss 1 (a = 0.001) the oracle's lerp rounds a differently in FP64
+        np.testing.assert_allclose(f1 - f0, 0.25 * 0.6 * gr * fr * dr, rtol=1e-9)
+
+
+def test_disney_specular_tint_closed_form():
+    """Cspec0 = lerp(spec 0.08 lerp(1, Ctint, st), Cdlin, m) (Disney.hpp:207):
+    at normal incidence Fs = Cspec0, Gs = 1/4 and Ds = 1 / (pi a^2), so
+    f(st=1) - f(st=0) = spec 0.08 (Ctint - 1) (1 - m) / (4 pi a^2)."""
+    alb, m, spec, r = np.array([0.9, 0.4, 0.1]), 0.2, 0.6, 0.4
+    ctint = alb / (0.3 * 0.9 + 0.6 * 0.4 + 0.1 * 0.1)
+    f1 = orc.disney_eval(_hd(albedo=alb, metallic=m, specular=spec, roughness=r, specularTint=1.0),
+                         (0, 0, -1), (0, 0, 1), 1)
+    f0 = orc.disney_eval(_hd(albedo=alb, metallic=m, specular=spec, roughness=r, specularTint=0.0),
+                         (0, 0, -1), (0, 0, 1), 1)
+    np.testing.assert_allclose(f1 - f0, spec * 0.08 * (ctint - 1) * (1 - m) / (4 * math.pi * r * r), rtol=1e-12)
+
+
+@pytest.mark.parametrize("aniso", [0.3, 0.9])
+def test_disney_anisotropic_lobe_against_isotropic(aniso):
+    """GTR2_aniso with H in the tangent plane's x-z section equals GTR2 with
+    a = ax scaled by ax / ay (1 + (a^2-1) h_z^2 = a^2 (h_x^2 / a^2 + h_z^2)),
+    and the anisotropic Smith term there equals the isotropic one at a = ax;
+    so with metallic 1 (no diffuse or sheen) and V = N, f(aniso, r) for L in
+    the T-N plane equals f(0, ax) ax / ay, and in the B-N plane f(0, ay)
+    ay / ax, with aspect = sqrt(1 - 0.9 aniso), ax = r / aspect, ay = r
+    aspect (Disney.hpp:224-231).  The basis for N = +z is T = +x, B = +y
+    (createBasis: up x N).  At normal incidence anisotropy changes nothing."""
+    r = 0.3
+    aspect = math.sqrt(1 - 0.9 * aniso)
+    ax, ay = r / aspect, r * aspect
+    alb = np.array([0.9, 0.7, 0.4])
+    hd = lambda an, rough: _hd(albedo=alb, metallic=1.0, roughness=rough, anisotropic=an)
+    for th in (0.3, 0.8, 1.2):
+        Lt = np.array([math.sin(th), 0.0, math.cos(th)])
+        Lb = np.array([0.0, math.sin(th), math.cos(th)])
+        np.testing.assert_allclose(orc.disney_eval(hd(aniso, r), (0, 0, -1), Lt, 1),
+                                   orc.disney_eval(hd(0.0, ax), (0, 0, -1), Lt, 1) * ax / ay, rtol=1e-11)
+        np.testing.assert_allclose(orc.disney_eval(hd(aniso, r), (0, 0, -1), Lb, 1),
+                                   orc.disney_eval(hd(0.0, ay), (0, 0, -1), Lb, 1) * ay / ax, rtol=1e-11)
+    np.testing.assert_allclose(orc.disney_eval(hd(aniso, r), (0, 0, -1), (0, 0, 1), 1),
+                               orc.disney_eval(hd(0.0, r), (0, 0, -1), (0, 0, 1), 1), rtol=1e-12)
+
+
 def _hemisphere_pdf_mass(hd, dirv, nb_theta=12, nb_phi=8, sub=24):
     """Integral of DisneyPdf over (cos theta, phi) bins of the upper hemisphere (midpoint rule)."""
     mass = np.zeros((nb_theta, nb_phi))
