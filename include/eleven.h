@@ -283,6 +283,23 @@ int  el_residual_l2(el_ctx* ctx, const float* target, double* loss);
  * call is ordered on the context's stream (still synchronous on return). */
 int  el_trace_closest(el_ctx* ctx, const el_ray* rays, uint32_t n,
                       int32_t* tri, float* t, float* uv, int device_ptrs);
+/* el_trace_closest for rays and outputs in device memory, allocating
+ * nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (tight serving loops, harnesses that forbid
+ * hipMalloc).  The traversal's scratch -- its tallies and the per-thread
+ * spill area of its stack -- is a static device array of the library,
+ * 16 MiB + 512 B on each GPU the library runs on, held until the process
+ * ends; the call allocates nothing, waits for its stream before returning
+ * (so it cannot be captured into a graph), and its results equal
+ * el_trace_closest's with device_ptrs = 1 (uv may be NULL).  The array
+ * bounds the resident threads to 16 MiB / (4 B x the scene's spill slots per
+ * thread), so large scenes trace with fewer threads than el_trace_closest.
+ * Every call on one GPU shares that array: calls that overlap on one GPU
+ * (two contexts on two host threads) race.  Fails with EL_ERR_RANGE when the
+ * scene's stack needs more spill slots per thread than the array holds for
+ * one block (65,536 at the default 64-thread block). */
+int  el_trace_closest_noalloc(el_ctx* ctx, const el_ray* d_rays, uint32_t n,
+                              int32_t* d_tri, float* d_t, float* d_uv);
 /* Any hit with 0 <= t < tmax[i]: occluded[i] = 1 / 0. */
 int  el_trace_any(el_ctx* ctx, const el_ray* rays, const float* tmax,
                   uint32_t n, uint8_t* occluded, int device_ptrs);

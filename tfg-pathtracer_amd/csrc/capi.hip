@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -67,8 +68,39 @@ inline unsigned long long tally(const unsigned long long* h, int slot) {
 
 }  // namespace
 
+// ---------------------------------------------------- static device scratch --
+// The scratch of el_trace_closest_noalloc: the tallies (kCounterWords 64-bit
+// words) and the traversal stack's per-thread spill area, one array per GPU
+// that lives until the process ends.  Host code never passes `scratch`
+// itself to a kernel (on the host it is only a stand-in for the symbol): each
+// device's address comes from hipGetSymbolAddress, looked up once per device
+// by el_create with that device current.
+constexpr size_t kScratchCounterBytes = 512;                  // >= kCounterWords * 8
+constexpr size_t kScratchSpillInts = size_t(4) << 20;         // 16 MiB of spill slots
+constexpr size_t kScratchBytes = kScratchCounterBytes + kScratchSpillInts * sizeof(int);
+static_assert(kScratchCounterBytes >= sizeof(unsigned long long) * kXcds * kStatSlots, "tallies fit");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratchMu;
+static std::vector<void*> g_scratchPtr;   // per device, null until looked up
+
+// Device address of `scratch` on the current device `device`.
+static hipError_t scratchAddress(int device, void** out) {
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if ((size_t)device >= g_scratchPtr.size()) g_scratchPtr.resize(device + 1, nullptr);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[device] = p;
+    }
+    *out = g_scratchPtr[device];
+    return hipSuccess;
+}
+
 struct el_ctx {
     int device = 0;
+    void* scratch = nullptr;       // this device's static scratch (scratchAddress)
     hipStream_t stream = nullptr;
     std::string err;
     int cus = 0;
@@ -177,6 +209,7 @@ int el_create(el_ctx** out, int device) {
     c->maxThreads = (uint32_t)c->cus * 2048u;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
     if (c->dCounters.ensure(sizeof(unsigned long long) * kCounterWords) != hipSuccess) return EL_ERR_HIP;
+    if (scratchAddress(device, &c->scratch) != hipSuccess) return EL_ERR_HIP;
     *out = c.release();
     return EL_OK;
 }
@@ -1049,6 +1082,33 @@ int el_trace_any(el_ctx* c, const el_ray* rays, const float* tmax, uint32_t n, u
 int el_trace_count(el_ctx* c, const el_ray* rays, uint32_t n, uint64_t* nodes, uint64_t* tris, int devp) {
     if (!nodes || !tris) return fail(c, EL_ERR_ARG, "null argument");
     return traceCommon(c, rays, nullptr, n, nullptr, nullptr, nullptr, nullptr, devp, false, true, nodes, tris);
+}
+
+int el_trace_closest_noalloc(el_ctx* c, const el_ray* d_rays, uint32_t n, int32_t* d_tri, float* d_t,
+                             float* d_uv) {
+    if (!c || !d_rays || !d_tri || !d_t) return fail(c, EL_ERR_ARG, "null argument");
+    if (!c->hasScene) return fail(c, EL_ERR_STATE, "no scene uploaded");
+    if (!c->scratch) return fail(c, EL_ERR_STATE, "no device scratch");
+    if (n == 0) return EL_OK;
+    EL_HIP(c, hipSetDevice(c->device));
+    // as many resident threads as the spill area holds at the scene's depth
+    LaunchCfg lc = cfgOf(c);
+    const uint64_t fit = kScratchSpillInts / (uint64_t)c->gcap;
+    uint32_t threads = (uint32_t)std::min<uint64_t>(c->maxThreads, fit);
+    threads -= threads % lc.block;
+    if (threads < lc.block)
+        return fail(c, EL_ERR_RANGE, "the scene's traversal stack (" + std::to_string(c->gcap) +
+                                         " spill slots per thread) exceeds the static scratch for one block");
+    lc.maxThreads = threads;
+    lc.gstack = (int*)((unsigned char*)c->scratch + kScratchCounterBytes);
+    unsigned long long* ctr = (unsigned long long*)c->scratch;
+    EL_HIP(c, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, c->stream));
+    EL_HIP(c, launchTrace(lc, c->ds, (const float*)d_rays, nullptr, n, d_tri, d_t, d_uv, nullptr, false, false, ctr));
+    unsigned long long h[kCounterWords];
+    EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    EL_HIP(c, hipStreamSynchronize(c->stream));
+    if (tally(h, kStatOverflow)) return fail(c, EL_ERR_RANGE, "traversal stack overflow (results incomplete)");
+    return EL_OK;
 }
 
 int el_trace_bench(el_ctx* c, const el_ray* d_rays, uint32_t n, int32_t* d_tri, float* d_t, int iters,

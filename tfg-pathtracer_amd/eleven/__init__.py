@@ -138,6 +138,7 @@ _SIGS = {
     "el_trace_any": (C.c_int, [_P, _P, _P, C.c_uint32, _P, C.c_int]),
     "el_trace_count": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.c_int]),
     "el_trace_bench": (C.c_int, [_P, _P, C.c_uint32, _P, _P, C.c_int, C.POINTER(C.c_double)]),
+    "el_trace_closest_noalloc": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P]),
     "el_device_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
     "el_device_free": (C.c_int, [_P, _P]),
     "el_memcpy_htod": (C.c_int, [_P, _P, _P, C.c_size_t]),
@@ -485,6 +486,10 @@ class Context:
 
     def dtoh(self, arr: np.ndarray, src):
         self._check(lib().el_memcpy_dtoh(self._h, arr.ctypes.data, src, arr.nbytes))
+
+    def trace_closest_noalloc(self, d_rays, n, d_tri, d_t, d_uv=None):
+        """el_trace_closest_noalloc: device pointers in and out, no allocation."""
+        self._check(lib().el_trace_closest_noalloc(self._h, d_rays, n, d_tri, d_t, d_uv))
 
     def trace_bench(self, d_rays, n, d_tri, d_t, iters):
         ms = C.c_double()
