@@ -349,17 +349,28 @@ def adjoint_bench(steps=5, with_cpu=True):
     return out
 
 
-def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
+TEX_BLOCKS = ((0, 64), (80, 32), (150, 16))   # c4 texture: coarse to fine (first step, texel block)
+INVERSE_EPS_REL = 0.0                          # c4 parameters' Adam epsilon / RMS of the first gradient
+
+
+def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_EPS_REL):
     """Multi-view inverse rendering at scale: the 1,048,576-triangle
     heightfield at 512x512 seen from 8 views, 16 spp forward + 16 spp replay
     per view, depth 6; Adam (lr 1e-2) on 16 materials x 6 parameters and the
     1024x1024 albedo map of material 0 (the other 15 materials have scalar
     albedo) from a seeded start: parameters perturbed by N(0, 0.1^2), texture
     flat 0.5.  Targets: the true parameters and texture rendered at 64 spp.
-    One gradient = all 8 views (forward + adjoint each); grads/s over the
-    Adam steps, parameter and texel RMSE before and after (texels also as
-    32x32-texel block means: material 0's tile covers ~128x64 pixels of a
-    view, so single texels are not observable).  Gradients are
+    The views survey the terrain (eleven.inverse.survey_views: a 4 x 2 grid
+    of cameras 22 units up, pitched 60 degrees, each over one column of the
+    4 x 4 material tiles and half the depth), so every material is seen by
+    62k-143k pixels and material 0's tile by 118k (the default frame's orbit
+    of 8 views saw material 0 in 9k pixels of one view and each material in
+    at most 12 of 16 views' frames).  A 512-pixel view resolves ~16 texels of
+    the 1024^2 map per pixel, so the texture is optimized coarse to fine as
+    block values (64-texel blocks from step 0, 32 from 80, 16 from 150;
+    eleven.inverse.optimize).  One gradient = all 8 views (forward + adjoint
+    each); grads/s over the Adam steps, parameter and texel RMSE before and
+    after (texels also as 32x32-texel block means).  Gradients are
     decorrelated (eleven.inverse): each view is rendered twice per step,
     once for the residual and once for the paths the adjoint replays."""
     import eleven
@@ -369,7 +380,7 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
     scene = eleven.SceneData.load(d)
     ctx = eleven.Context(0)
     ctx.upload(scene)
-    cams = inverse.orbit_views(scene.camera, views, x_span=16.0)
+    cams = inverse.survey_views(scene.camera, res=(res, res))[:views]
     mv = inverse.MultiView(ctx, scene, cams, spp=spp, depth=depth, grad_texture=0)
     truth = inverse.material_params(scene)
     tex_truth = scene.texture(0).copy()
@@ -378,7 +389,8 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
     p0 = np.clip(truth + rng.normal(0, 0.1, truth.shape).astype(np.float32), inverse.PARAM_LO, inverse.PARAM_HI)
     t0 = np.full_like(tex_truth, 0.5)
     _, g0, _ = mv.loss_and_grad(p0, t0, targets, seed=1)
-    res_ = inverse.optimize(mv, p0, t0, targets, steps=steps, lr=1e-2, log=log)
+    res_ = inverse.optimize(mv, p0, t0, targets, steps=steps, lr=1e-2, log=log, tex_blocks=TEX_BLOCKS,
+                            eps_rel=eps_rel)
     mask = np.ones(truth.shape, bool)
     mask[0, :3] = False   # material 0's scalar albedo is replaced by its texture
 
@@ -387,12 +399,18 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6):
         return t.reshape(h // k, k, w // k, k, c).mean(axis=(1, 3))
 
     out = {"metric": "inverse grads/s", "value": round(res_.grads_per_s, 3), "unit": "grads/s",
-           "config": f"heightfield 1,048,576 tris, {res}x{res}, {views} views, {spp} spp forward + {spp} spp "
-                     f"replay, depth {depth}, Adam lr 1e-2, 16 materials x 6 parameters + 1024x1024 albedo",
-           "steps": steps, "ms_per_grad": round(res_.seconds / steps * 1e3, 2),
+           "config": f"heightfield 1,048,576 tris, {res}x{res}, {views} survey views, {spp} spp forward + {spp} "
+                     f"spp replay, depth {depth}, Adam lr 1e-2, 16 materials x 6 parameters + 1024x1024 albedo "
+                     f"(texel blocks {TEX_BLOCKS})",
+           "steps": steps, "adam_eps_rel": eps_rel, "ms_per_grad": round(res_.seconds / steps * 1e3, 2),
            "loss_first5": float(np.mean(res_.losses[:5])), "loss_last5": float(np.mean(res_.losses[-5:])),
            "param_rmse": {"init": round(inverse.param_rmse(p0, truth, mask), 5),
                           "final": round(inverse.param_rmse(res_.params, truth, mask), 5)},
+           "param_rmse_by_kind": {name: {"init": round(inverse.param_rmse(p0[:, cols], truth[:, cols], mask[:, cols]), 5),
+                                         "final": round(inverse.param_rmse(res_.params[:, cols], truth[:, cols],
+                                                                           mask[:, cols]), 5)}
+                                  for name, cols in (("albedo", slice(0, 3)), ("roughness", slice(3, 4)),
+                                                     ("metallic", slice(4, 5)), ("specular", slice(5, 6)))},
            "param_rmse_gradient_weighted": {"init": round(inverse.param_rmse(p0, truth, mask, g0), 5),
                                             "final": round(inverse.param_rmse(res_.params, truth, mask, g0), 5)},
            "texel_rmse": {"init": round(inverse.param_rmse(t0, tex_truth), 5),
@@ -485,6 +503,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--inverse-steps", type=int, default=200)
+    ap.add_argument("--inverse-eps-rel", type=float, default=INVERSE_EPS_REL)
     ap.add_argument("--copy-bw", action="store_true",
                     help="only measure the achievable copy bandwidth (JSON line); the roofline leg runs this "
                          "in a child process started before this process touches the GPU")
@@ -668,7 +687,7 @@ def main():
         out["batch"] = batch_bench(with_plain=not args.no_plain)
     if not args.no_inverse:
         log("inverse ...")
-        out["inverse"] = inverse_bench(steps=args.inverse_steps)
+        out["inverse"] = inverse_bench(steps=args.inverse_steps, eps_rel=args.inverse_eps_rel)
     print(json.dumps(out), flush=True)
 
 

@@ -30,6 +30,17 @@ namespace dev {
 
 #define EL_PI 3.14159265358979323846f
 
+// Traversal variants (compile-time).
+#ifndef EL_LEAF_PIPE   // leaf triangles: next record loaded before the current one is tested
+#define EL_LEAF_PIPE 1
+#endif
+#ifndef EL_SLAB_AB     // slab distances as A + q B per axis (2 VALU per plane) instead of (b - o) / d
+#define EL_SLAB_AB 1
+#endif
+#ifndef EL_PUSH_BULK   // one stack-room check per node visit instead of one per push
+#define EL_PUSH_BULK 0
+#endif
+
 // ---------------------------------------------------------------- vectors --
 struct v3 {
     float x, y, z;
@@ -709,6 +720,23 @@ struct TravStack {
         lds[(sp & (kShortStack - 1)) * ldsStride] = e;
         sp++;
     }
+    // Room for k more pushes (k <= kShortStack): the oldest LDS entries go
+    // to the global area first, as push would spill them one at a time.
+    __device__ __forceinline__ void reserve(int k) {
+        while (sp - lo + k > kShortStack) {
+            int old = lds[(lo & (kShortStack - 1)) * ldsStride];
+            if (lo < gcap)
+                g[(uint64_t)lo * gstride] = old;
+            else
+                overflow = true;
+            lo++;
+        }
+    }
+    // push after reserve: no spill check
+    __device__ __forceinline__ void pushReserved(int e) {
+        lds[(sp & (kShortStack - 1)) * ldsStride] = e;
+        sp++;
+    }
     __device__ __forceinline__ int pop() {
         if (sp == 0) return -1;
         sp--;
@@ -803,6 +831,35 @@ __device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, floa
         farZ[d] = nz ? q[4 * D + d] : q[5 * D + d];
     }
     uint32_t keys[W];
+#if EL_SLAB_AB
+    // Plane b = p + q s, so (b - o) / d = A + q B per axis with
+    // A = (p - o) inv and B = s inv (exact: s is a power of two), and
+    // fma(q, B, A) rounds once; A carries two roundings, so a distance is off
+    // by at most 2^-23 |A| + 2^-24 |t|.  Near planes use A - E and far planes
+    // A + E with E = 2^-19 |A| of their own axis (an axis whose direction is
+    // near zero has a huge A, but also huge distances of its own), and the
+    // 2^-20 |tf| widening below covers the |t| term: the test stays
+    // conservative.
+    const float Ax = (px - ray.ox) * ray.ix, Ay = (py - ray.oy) * ray.iy, Az = (pz - ray.oz) * ray.iz;
+    const float Bx = sx * ray.ix, By = sy * ray.iy, Bz = sz * ray.iz;
+    const float Ex = fabsf(Ax) * (1.0f / 524288.0f), Ey = fabsf(Ay) * (1.0f / 524288.0f),
+                Ez = fabsf(Az) * (1.0f / 524288.0f);
+    const float Anx = Ax - Ex, Afx = Ax + Ex, Any = Ay - Ey, Afy = Ay + Ey, Anz = Az - Ez, Afz = Az + Ez;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        const int d = i >> 2, sh = 8 * (i & 3);
+        const float tx0 = fmaf((float)((nearX[d] >> sh) & 0xffu), Bx, Anx);
+        const float tx1 = fmaf((float)((farX[d] >> sh) & 0xffu), Bx, Afx);
+        const float ty0 = fmaf((float)((nearY[d] >> sh) & 0xffu), By, Any);
+        const float ty1 = fmaf((float)((farY[d] >> sh) & 0xffu), By, Afy);
+        const float tz0 = fmaf((float)((nearZ[d] >> sh) & 0xffu), Bz, Anz);
+        const float tz1 = fmaf((float)((farZ[d] >> sh) & 0xffu), Bz, Afz);
+        const float tn = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, 0.0f));
+        float tf = fminf(fminf(tx1, ty1), fminf(tz1, tcull));
+        tf = fmaf(fabsf(tf), 1.0f / 1048576.0f, tf);
+        keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+    }
+#else
 #pragma unroll
     for (int i = 0; i < W; i++) {
         const int d = i >> 2, sh = 8 * (i & 3);
@@ -818,12 +875,23 @@ __device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, floa
         tf = fmaf(fabsf(tf), 1.0f / 1048576.0f, tf);
         keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
     }
+#endif
     if constexpr (UNSORTED) {
         int next = -2;   // the lowest hit slot is visited next, the others are pushed
+#if EL_PUSH_BULK
+        int nhit = 0;
+#pragma unroll
+        for (int i = 0; i < W; i++) nhit += keys[i] != 0xFFFFFFFFu;
+        if (nhit > 1) st.reserve(nhit - 1);
+#endif
 #pragma unroll
         for (int i = W - 1; i >= 0; i--) {
             if (keys[i] == 0xFFFFFFFFu) continue;
+#if EL_PUSH_BULK
+            if (next != -2) st.pushReserved(child[next]);
+#else
             if (next != -2) st.push(child[next]);
+#endif
             next = i;
         }
         return next == -2 ? st.pop() : child[next];
@@ -849,9 +917,16 @@ __device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, floa
 #pragma unroll
     for (int i = 0; i < W; i++) nh += keys[i] != 0xFFFFFFFFu;
     if (nh == 0) return st.pop();
+#if EL_PUSH_BULK
+    if (nh > 1) st.reserve(nh - 1);
+#pragma unroll
+    for (int i = W - 1; i >= 1; i--)
+        if (i < nh) st.pushReserved(child[keys[i] & 7u]);
+#else
 #pragma unroll
     for (int i = W - 1; i >= 1; i--)
         if (i < nh) st.push(child[keys[i] & 7u]);
+#endif
     return child[keys[0] & 7u];
 }
 
@@ -869,14 +944,32 @@ __device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cu
 // Tests the triangles of leaf `code` (<= -2) and updates the closest hit:
 // smaller t wins, equal t goes to the lower original index.  For ANY the
 // first hit below r.t is taken and true is returned.
+//
+// The leaf's triangles are contiguous records; the three words of the next
+// triangle are loaded before the current one is tested, and all three
+// words of a triangle together (the vertex word is otherwise fetched only
+// after the determinant test), so a leaf costs about one memory round trip
+// instead of two per triangle.
 template <bool ANY, bool COUNT>
 __device__ __forceinline__ bool testLeaf(const float4* __restrict__ tris, int code, const TRay& ray, TraceResult& r,
                                          TraceCounters& cnt) {
     int k = -code - 2;
     int first = k >> 4, count = (k & 15) + 1;
+    const float4* p = tris + (uint64_t)first * 3;
+#if EL_LEAF_PIPE
+    float4 nA = p[0], nB = p[1], nC = p[2];
+#endif
     for (int i = 0; i < count; i++) {
-        const float4* p = tris + (uint64_t)(first + i) * 3;
-        float4 A = p[0], B = p[1], C = p[2];
+#if EL_LEAF_PIPE
+        const float4 A = nA, B = nB, C = nC;
+        if (i + 1 < count) {
+            nA = p[3 * (i + 1)];
+            nB = p[3 * (i + 1) + 1];
+            nC = p[3 * (i + 1) + 2];
+        }
+#else
+        float4 A = p[3 * i], B = p[3 * i + 1], C = p[3 * i + 2];
+#endif
         if (COUNT) cnt.tris++;
         float t, u, v;
         if (mtHit(A, B, C, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, t, u, v)) {

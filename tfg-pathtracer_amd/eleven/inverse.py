@@ -55,6 +55,29 @@ def orbit_views(base: el_camera, n: int, x_span: float, pitches=(29.0, 35.0), re
     return out
 
 
+def survey_views(base: el_camera, extent=(51.2, 25.6), columns: int = 4, pitch: float = 60.0,
+                 height: float = 22.0, res=None) -> list:
+    """2 x `columns` cameras over a terrain spanning [-ex, ex] x [-ez, ez]
+    (x, z): each at `height` above the ground plane, pitched down by
+    `pitch` degrees and aimed at the centre of its column, the near half
+    (z = -0.75 ez) or the far half (z = 0.5 ez).  With the 35 mm lens on a
+    square frame (53 degrees across) and 60 degrees of pitch at height 22,
+    a view spans one column of the 4 x 4 material tiles of a 102.4 x 51.2
+    terrain and about half its depth, so the 8 views see every tile."""
+    out = []
+    ex, ez = extent
+    back = height / np.tan(np.radians(pitch))
+    for zc in (-0.75 * ez, 0.5 * ez):
+        for i in range(columns):
+            xc = -ex + (2 * i + 1) * ex / columns
+            xr, yr = res if res else (base.x_res, base.y_res)
+            out.append(make_camera(xr, yr, position=(xc, height, zc - back), rotation=(pitch, 0.0, 0.0),
+                                   focal_length=base.focal_length, sensor_width=base.sensor_width,
+                                   aperture=base.aperture, focus_distance=float(np.hypot(height, back)),
+                                   bokeh=bool(base.bokeh)))
+    return out
+
+
 class MultiView:
     """Forward renders and summed adjoint gradients of one uploaded scene
     seen from several cameras."""
@@ -118,28 +141,74 @@ class InverseResult:
         return self.steps / self.seconds if self.seconds > 0 else 0.0
 
 
+def block_up(c: np.ndarray, k: int) -> np.ndarray:
+    """Texture of k x k texel blocks from its block values c [h][w][3]."""
+    return c if k == 1 else np.repeat(np.repeat(c, k, axis=0), k, axis=1)
+
+
+def block_sum(t: np.ndarray, k: int) -> np.ndarray:
+    """Adjoint of block_up: the sum over each k x k block."""
+    if k == 1:
+        return t
+    h, w, ch = t.shape
+    return t.reshape(h // k, k, w // k, k, ch).sum(axis=(1, 3))
+
+
+def block_mean(t: np.ndarray, k: int) -> np.ndarray:
+    h, w, ch = t.shape
+    return t if k == 1 else t.reshape(h // k, k, w // k, k, ch).mean(axis=(1, 3))
+
+
 def optimize(mv: MultiView, params0, texture0, targets, steps: int = 200, lr: float = 1e-2, seed: int = 1000,
-             log=None) -> InverseResult:
+             log=None, tex_blocks=((0, 1),), eps_rel: float = 0.0) -> InverseResult:
     """Adam (torch.optim) on the parameters and the texture; one gradient
-    vector (all views) per step."""
+    vector (all views) per step.
+
+    eps_rel > 0: the material parameters' Adam epsilon is eps_rel times the
+    RMS of the first gradient.  Adam divides a parameter's mean gradient by
+    its RMS, so a parameter the images barely see (a tiny, noise-dominated
+    gradient) still takes steps of about lr and random-walks; an epsilon at
+    the scale of the gradients keeps such a parameter nearly still while
+    parameters with gradients well above it move as before.
+
+    tex_blocks: (first step, k) pairs, coarse to fine.  From its first step
+    on, the texture is parameterized by one value per k x k texel block
+    (the texture is those values repeated over their blocks; the gradient of
+    a block value is the sum of its texels' gradients), starting from the
+    block means of the texture so far; the optimizer state of the texture
+    restarts at each change of k.  ((0, 1),) optimizes every texel."""
     p = torch.tensor(np.asarray(params0, np.float32).copy(), requires_grad=True)
-    tensors = [p]
-    t = None
+    sched = sorted(tex_blocks)
     if texture0 is not None:
-        t = torch.tensor(np.asarray(texture0, np.float32).copy(), requires_grad=True)
-        tensors.append(t)
-    opt = torch.optim.Adam(tensors, lr=lr)
+        h, w, _ = np.shape(texture0)
+        for _, k in sched:
+            if k < 1 or h % k or w % k:
+                raise ValueError(f"texture block {k} does not divide the {w}x{h} texture")
+    opt_p = None
+    tex = None if texture0 is None else np.asarray(texture0, np.float32).copy()
+    t, opt_t, kcur = None, None, None
     lo, hi = torch.from_numpy(PARAM_LO), torch.from_numpy(PARAM_HI)
     res = InverseResult(params=None, texture=None)
     t0 = time.perf_counter()
     for k in range(steps):
-        loss, dmat, dtex = mv.loss_and_grad(p.detach().numpy(), None if t is None else t.detach().numpy(), targets,
-                                            seed + k * len(mv.cameras))
-        opt.zero_grad()
+        if tex is not None:
+            kk = [b for s0, b in sched if s0 <= k][-1]
+            if kk != kcur:   # a new (finer) parameterization from the texture so far
+                cur = tex if t is None else block_up(t.detach().numpy(), kcur)
+                t = torch.tensor(block_mean(cur, kk).astype(np.float32), requires_grad=True)
+                opt_t, kcur = torch.optim.Adam([t], lr=lr), kk
+            tex = block_up(t.detach().numpy(), kcur)
+        loss, dmat, dtex = mv.loss_and_grad(p.detach().numpy(), tex, targets, seed + k * len(mv.cameras))
+        if opt_p is None:
+            eps = eps_rel * float(np.sqrt(np.mean(np.square(dmat, dtype=np.float64)))) if eps_rel > 0 else 1e-8
+            opt_p = torch.optim.Adam([p], lr=lr, eps=max(eps, 1e-8))
+        opt_p.zero_grad()
         p.grad = torch.from_numpy(dmat.astype(np.float32))
+        opt_p.step()
         if t is not None:
-            t.grad = torch.from_numpy(dtex.astype(np.float32))
-        opt.step()
+            opt_t.zero_grad()
+            t.grad = torch.from_numpy(block_sum(dtex.astype(np.float32), kcur))
+            opt_t.step()
         with torch.no_grad():
             p.copy_(torch.max(torch.min(p, hi), lo))
             if t is not None:
@@ -150,7 +219,7 @@ def optimize(mv: MultiView, params0, texture0, targets, steps: int = 200, lr: fl
     res.seconds = time.perf_counter() - t0
     res.steps = steps
     res.params = p.detach().numpy().copy()
-    res.texture = None if t is None else t.detach().numpy().copy()
+    res.texture = None if t is None else block_up(t.detach().numpy(), kcur).copy()
     return res
 
 
@@ -164,5 +233,5 @@ def param_rmse(a: np.ndarray, b: np.ndarray, mask: np.ndarray | None = None, wei
     return float(np.sqrt((w * d).sum() / w.sum()))
 
 
-__all__ = ["MultiView", "InverseResult", "optimize", "orbit_views", "param_rmse", "material_params",
-           "PARAM_LO", "PARAM_HI"]
+__all__ = ["MultiView", "InverseResult", "optimize", "orbit_views", "survey_views", "param_rmse", "material_params",
+           "block_up", "block_sum", "block_mean", "PARAM_LO", "PARAM_HI"]
