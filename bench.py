@@ -350,10 +350,11 @@ def adjoint_bench(steps=5, with_cpu=True):
 
 
 TEX_BLOCKS = ((0, 64), (80, 32), (150, 16))   # c4 texture: coarse to fine (first step, texel block)
-INVERSE_EPS_REL = 0.0                          # c4 parameters' Adam epsilon / RMS of the first gradient
+INVERSE_EPS_REL = 0.3                          # c4 parameters' Adam epsilon / RMS of the first gradient
+INVERSE_TARGET_SPP = 64                        # samples per pixel of the c4 target images
 
 
-def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_EPS_REL):
+def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_EPS_REL, target_spp=INVERSE_TARGET_SPP):
     """Multi-view inverse rendering at scale: the 1,048,576-triangle
     heightfield at 512x512 seen from 8 views, 16 spp forward + 16 spp replay
     per view, depth 6; Adam (lr 1e-2) on 16 materials x 6 parameters and the
@@ -370,7 +371,10 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_
     block values (64-texel blocks from step 0, 32 from 80, 16 from 150;
     eleven.inverse.optimize).  One gradient = all 8 views (forward + adjoint
     each); grads/s over the Adam steps, parameter and texel RMSE before and
-    after (texels also as 32x32-texel block means).  Gradients are
+    after (texels also as 32x32-texel block means).  The material
+    parameters' Adam epsilon is 0.3 x the RMS of the first gradient, so
+    parameters the images barely see (specular of rough dielectrics) do not
+    random-walk on gradient noise.  Gradients are
     decorrelated (eleven.inverse): each view is rendered twice per step,
     once for the residual and once for the paths the adjoint replays."""
     import eleven
@@ -384,7 +388,7 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_
     mv = inverse.MultiView(ctx, scene, cams, spp=spp, depth=depth, grad_texture=0)
     truth = inverse.material_params(scene)
     tex_truth = scene.texture(0).copy()
-    targets = [im.astype(np.float64) for im in mv.images(truth, tex_truth, seed=777, spp=64)]
+    targets = [im.astype(np.float64) for im in mv.images(truth, tex_truth, seed=777, spp=target_spp)]
     rng = np.random.default_rng(3)
     p0 = np.clip(truth + rng.normal(0, 0.1, truth.shape).astype(np.float32), inverse.PARAM_LO, inverse.PARAM_HI)
     t0 = np.full_like(tex_truth, 0.5)
@@ -402,7 +406,8 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_
            "config": f"heightfield 1,048,576 tris, {res}x{res}, {views} survey views, {spp} spp forward + {spp} "
                      f"spp replay, depth {depth}, Adam lr 1e-2, 16 materials x 6 parameters + 1024x1024 albedo "
                      f"(texel blocks {TEX_BLOCKS})",
-           "steps": steps, "adam_eps_rel": eps_rel, "ms_per_grad": round(res_.seconds / steps * 1e3, 2),
+           "steps": steps, "adam_eps_rel": eps_rel, "target_spp": target_spp,
+           "ms_per_grad": round(res_.seconds / steps * 1e3, 2),
            "loss_first5": float(np.mean(res_.losses[:5])), "loss_last5": float(np.mean(res_.losses[-5:])),
            "param_rmse": {"init": round(inverse.param_rmse(p0, truth, mask), 5),
                           "final": round(inverse.param_rmse(res_.params, truth, mask), 5)},
@@ -413,6 +418,10 @@ def inverse_bench(steps=200, views=8, res=512, spp=16, depth=6, eps_rel=INVERSE_
                                                      ("metallic", slice(4, 5)), ("specular", slice(5, 6)))},
            "param_rmse_gradient_weighted": {"init": round(inverse.param_rmse(p0, truth, mask, g0), 5),
                                             "final": round(inverse.param_rmse(res_.params, truth, mask, g0), 5)},
+           "specular": {"truth": [round(float(x), 3) for x in truth[:, 5]],
+                        "final": [round(float(x), 3) for x in res_.params[:, 5]]},
+           "metallic": {"truth": [round(float(x), 3) for x in truth[:, 4]],
+                        "final": [round(float(x), 3) for x in res_.params[:, 4]]},
            "texel_rmse": {"init": round(inverse.param_rmse(t0, tex_truth), 5),
                           "final": round(inverse.param_rmse(res_.texture, tex_truth), 5)},
            "texture_rmse_32x32_block_means": {"init": round(inverse.param_rmse(blocks(t0), blocks(tex_truth)), 5),
@@ -504,6 +513,7 @@ def main():
     ap.add_argument("--no-inverse", action="store_true")
     ap.add_argument("--inverse-steps", type=int, default=200)
     ap.add_argument("--inverse-eps-rel", type=float, default=INVERSE_EPS_REL)
+    ap.add_argument("--inverse-target-spp", type=int, default=INVERSE_TARGET_SPP)
     ap.add_argument("--copy-bw", action="store_true",
                     help="only measure the achievable copy bandwidth (JSON line); the roofline leg runs this "
                          "in a child process started before this process touches the GPU")
@@ -687,7 +697,8 @@ def main():
         out["batch"] = batch_bench(with_plain=not args.no_plain)
     if not args.no_inverse:
         log("inverse ...")
-        out["inverse"] = inverse_bench(steps=args.inverse_steps, eps_rel=args.inverse_eps_rel)
+        out["inverse"] = inverse_bench(steps=args.inverse_steps, eps_rel=args.inverse_eps_rel,
+                                       target_spp=args.inverse_target_spp)
     print(json.dumps(out), flush=True)
 
 

@@ -147,7 +147,11 @@ typedef struct {
     uint64_t paths;            /* paths traced                               */
     double   render_ms;        /* wall time of the last el_render (device)   */
     double   extend_ms;        /* summed closest-hit kernel time (wavefront) */
-    double   shadow_ms;        /* summed shadow kernel time (wavefront)      */
+    double   shadow_ms;        /* summed shadow kernel time (wavefront); el_render
+                                  runs a bounce's shadow launches on a second
+                                  stream while the next bounce's closest-hit
+                                  launch runs, so the three times overlap and
+                                  their sum exceeds render_ms                */
     double   shade_ms;         /* summed shading kernel time (wavefront)     */
     uint64_t extend_launches;  /* number of closest-hit launches timed       */
     uint64_t extend_rays;      /* rays processed by those launches           */

@@ -146,13 +146,13 @@ static void damage(const fs::path& src, const fs::path& scratch) {
         if (good.empty()) continue;
         // truncations: a spread of lengths, denser near the start (headers)
         std::vector<size_t> cuts = {0, 1, 2, 8, 14, 18, 26, 34, 54, 60, 100};
-        for (int k = 1; k < 24; k++) cuts.push_back(good.size() * k / 24);
+        for (int k = 1; k < 12; k++) cuts.push_back(good.size() * k / 12);
         for (size_t c : cuts)
             if (c < good.size()) tryDamaged(src, work, name, std::vector<char>(good.begin(), good.begin() + c));
         // overwritten bytes (headers first, then anywhere)
-        for (int it = 0; it < 40; it++) {
+        for (int it = 0; it < 24; it++) {
             std::vector<char> b = good;
-            const size_t span = it < 20 ? std::min<size_t>(b.size(), 64) : b.size();
+            const size_t span = it < 12 ? std::min<size_t>(b.size(), 64) : b.size();
             for (int j = 0; j < 1 + (int)(g() % 4); j++) b[g() % span] = (char)(g() & 0xff);
             tryDamaged(src, work, name, b);
         }

@@ -52,7 +52,7 @@ def test_host_code_under_asan_and_ubsan(tmp_path):
     if b.returncode != 0 and "asan" in b.stderr.lower() and "cannot find" in b.stderr.lower():
         pytest.skip("this toolchain has no sanitizer runtime")
     assert b.returncode == 0, b.stderr[-4000:]
-    scenes = [   # the first is the one damaged: small, so its 300-odd loads stay quick
+    scenes = [   # the first is the one damaged: small, so its ~200 loads stay quick
         scenegen.sphere_field(tmp_path / "sf", n_spheres=3, res=(8, 8), seed=3, seg=8, rings=6, ground_tex_size=16),
         scenegen.heightfield(tmp_path / "hf", cells=(32, 16), extent=(6.4, 3.2), res=(16, 9), n_mats=3,
                              tex_size=32, sky=(64, 32)),

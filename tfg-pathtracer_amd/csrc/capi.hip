@@ -148,6 +148,11 @@ struct el_ctx {
     bool aovOn = false;
 
     std::vector<hipEvent_t> evPool;
+    // Wavefront renders: the shadow launches of bounce b run on `shadowStream`
+    // while the extension launch of bounce b + 1 runs on `stream`; syncPool
+    // holds the events that order them (no timing).
+    hipStream_t shadowStream = nullptr;
+    std::vector<hipEvent_t> syncPool;
 };
 
 static int fail(el_ctx* c, int code, const std::string& what) {
@@ -208,6 +213,7 @@ int el_create(el_ctx** out, int device) {
     // resident-thread capacity: 32 waves of 64 lanes per CU
     c->maxThreads = (uint32_t)c->cus * 2048u;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c->shadowStream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
     if (c->dCounters.ensure(sizeof(unsigned long long) * kCounterWords) != hipSuccess) return EL_ERR_HIP;
     if (scratchAddress(device, &c->scratch) != hipSuccess) return EL_ERR_HIP;
     *out = c.release();
@@ -218,8 +224,11 @@ void el_destroy(el_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->shadowStream) (void)hipStreamSynchronize(c->shadowStream);
     for (auto e : c->evPool) (void)hipEventDestroy(e);
+    for (auto e : c->syncPool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->shadowStream) (void)hipStreamDestroy(c->shadowStream);
     delete c;
 }
 
@@ -538,6 +547,15 @@ int el_clear(el_ctx* c) {
 
 }  // extern "C"
 
+static hipEvent_t syncEvent(el_ctx* c, size_t i) {
+    while (c->syncPool.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        c->syncPool.push_back(e);
+    }
+    return c->syncPool[i];
+}
+
 static hipEvent_t poolEvent(el_ctx* c, size_t i) {
     while (c->evPool.size() <= i) {
         hipEvent_t e;
@@ -725,8 +743,18 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
             a.genSample = (uint32_t)(start / npad);
             a.genCount = (uint32_t)std::min<uint64_t>(Q, total - start);
             EL_HIP(c, launchWfCamera(lc, c->ds, a, q));
+            // Per bounce b: extend (stream) -> shade (stream) -> both shadow
+            // launches (shadowStream).  The shadows of bounce b need only
+            // what shade b wrote and update nothing the extension of bounce
+            // b + 1 reads (their counters are set b & 1, the extension
+            // clears set (b + 1) & 1), so the two overlap; shade b + 1 and
+            // finalize wait for them (they add to the path radiance).
+            LaunchCfg lcs = lc;
+            lcs.stream = c->shadowStream;
+            hipEvent_t shadowsDone = nullptr;
             for (uint32_t b = 0; b < depth; b++) {
                 a.bounce = b;
+                q.parity = b & 1u;
                 const int cur = b & 1, nxt = cur ^ 1;
                 uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
                 uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
@@ -736,24 +764,34 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                 EL_HIP(c, hipEventRecord(x1, c->stream));
                 timed.push_back({ev, 0});
                 ev += 2;
+                if (shadowsDone) EL_HIP(c, hipStreamWaitEvent(c->stream, shadowsDone, 0));
                 hipEvent_t y0 = poolEvent(c, ev), y1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(y0, c->stream));
                 EL_HIP(c, launchWfShade(lc, c->ds, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, a.genCount, ctr));
                 EL_HIP(c, hipEventRecord(y1, c->stream));
                 timed.push_back({ev, 1});
                 ev += 2;
+                hipEvent_t shaded = syncEvent(c, 2 * b), done = syncEvent(c, 2 * b + 1);
+                if (!shaded || !done) return fail(c, EL_ERR_HIP, "event creation failed");
+                EL_HIP(c, hipEventRecord(shaded, c->stream));
+                EL_HIP(c, hipStreamWaitEvent(c->shadowStream, shaded, 0));
                 hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
-                EL_HIP(c, hipEventRecord(z0, c->stream));
+                EL_HIP(c, hipEventRecord(z0, c->shadowStream));
                 if (recordHits) q.visBit = 1u << (2 * b);
-                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowH, q.counters + kQShadowH, a.genCount, ctr));
+                EL_HIP(c, launchWfShadow(lcs, c->ds, q, q.shadowH, q.counters + qShadow(q.parity, 0), a.genCount,
+                                         ctr));
                 if (recordHits) q.visBit = 1u << (2 * b + 1);
-                EL_HIP(c, launchWfShadow(lc, c->ds, q, q.shadowP, q.counters + kQShadowP, a.genCount, ctr));
+                EL_HIP(c, launchWfShadow(lcs, c->ds, q, q.shadowP, q.counters + qShadow(q.parity, 1), a.genCount,
+                                         ctr));
                 q.visBit = 0;
-                EL_HIP(c, hipEventRecord(z1, c->stream));
+                EL_HIP(c, hipEventRecord(z1, c->shadowStream));
+                EL_HIP(c, hipEventRecord(done, c->shadowStream));
+                shadowsDone = done;
                 timed.push_back({ev, 2});
                 ev += 2;
                 extendLaunches++;
             }
+            if (shadowsDone) EL_HIP(c, hipStreamWaitEvent(c->stream, shadowsDone, 0));
             EL_HIP(c, launchWfFinalize(lc, c->ds, a, q, c->dAccum.as<float4>(), c->dCounts.as<uint32_t>(), aovBuf,
                                        ctr));
         }
@@ -1446,16 +1484,17 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
                 const int cur = bo & 1, nxt = cur ^ 1;
                 uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
                 uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
+                q.parity = bo & 1u;
                 if ((e = launchBtExtend(lc, dsc, width, q, q.ray[cur], cntCur, cntNxt, g.count, ctr)) != hipSuccess)
                     return e;
                 if ((e = launchBtShade(lc, dsc, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, g.count, ctr)) !=
                     hipSuccess)
                     return e;
-                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowH, q.counters + kQShadowH, g.count, ctr)) !=
-                    hipSuccess)
+                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowH, q.counters + qShadow(q.parity, 0), g.count,
+                                        ctr)) != hipSuccess)
                     return e;
-                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowP, q.counters + kQShadowP, g.count, ctr)) !=
-                    hipSuccess)
+                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowP, q.counters + qShadow(q.parity, 1), g.count,
+                                        ctr)) != hipSuccess)
                     return e;
             }
             if ((e = launchBtFinalize(lc, dsc, gseg, g.nseg, g.finTotal, q, ctr)) != hipSuccess) return e;

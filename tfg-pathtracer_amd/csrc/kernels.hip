@@ -808,13 +808,16 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
     QueueMap qm;
     qm.load(countIn);
     const uint32_t n = qm.total();
+    // the counters the shade and shadow launches of this bounce use (set
+    // q.parity; the shadow launches of the previous bounce, which may still
+    // run, use the other set)
     if (blockIdx.x == 0 && threadIdx.x < kXcds) {
         resetNext[threadIdx.x] = 0;
-        q.counters[kQShadowH + threadIdx.x] = 0;
-        q.counters[kQShadowP + threadIdx.x] = 0;
+        q.counters[qShadow(q.parity, 0) + threadIdx.x] = 0;
+        q.counters[qShadow(q.parity, 1) + threadIdx.x] = 0;
     }
-    resetChunks(q.counters, kChunkShH);   // for the shadow launches that follow
-    resetChunks(q.counters, kChunkShP);
+    resetChunks(q.counters, chunkShadow(q.parity, 0));
+    resetChunks(q.counters, chunkShadow(q.parity, 1));
     uint32_t rays = 0;
     const WorkSource ws = chunkSource<P::kMany>(q.counters, kChunkExt, n);
     const uint32_t segSize = q.segSize;
@@ -880,7 +883,7 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
     uint32_t rays = 0, verts = 0, overflow = 0;
     const int bounce = (int)a.bounce;
     const bool last = bounce + 1 >= (int)a.maxDepth;
-    uint32_t* const outCnt[3] = {countOut, q.counters + kQShadowH, q.counters + kQShadowP};
+    uint32_t* const outCnt[3] = {countOut, q.counters + qShadow(q.parity, 0), q.counters + qShadow(q.parity, 1)};
     const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6, lane = __lane_id();
     const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
     resetChunks(q.counters, kChunkExt);   // for the next bounce's extend launch
@@ -1036,7 +1039,7 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
     qm.load(countIn);
     const uint32_t n = qm.total();
     uint32_t traced = 0;
-    const WorkSource ws = chunkSource<P::kMany>(q.counters, sh == q.shadowH ? kChunkShH : kChunkShP, n);
+    const WorkSource ws = chunkSource<P::kMany>(q.counters, chunkShadow(q.parity, sh == q.shadowH ? 0 : 1), n);
     const uint32_t segSize = q.segSize;
     auto load = [&](uint32_t i, float& ox, float& oy, float& oz, float& dx, float& dy, float& dz, float& tm,
                     BvhPtr& bvh) {

@@ -30,15 +30,20 @@ constexpr int kStatSlots = 8;
 constexpr int kStatRays = 0, kStatVerts = 1, kStatPaths = 2, kStatOverflow = 3, kStatNodes = 4, kStatTris = 5,
               kStatExtRays = 6;   // rays of the closest-hit (extension) launches
 
-// Queue counters: kXcds per queue.
-constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadowH = 2 * kXcds, kQShadowP = 3 * kXcds;
-// Work-chunk counters of the traversal launches (extend, HDRI shadow, light
-// shadow), each on its own 128-byte line: a batch launch draws from
-// kChunkShares shares (share x first by the blocks b with b % 64 == x), a
-// single-scene launch from the first kXcds (one per XCD).
+// Queue counters: kXcds per queue.  The shadow queues' counts come in two
+// sets by bounce parity (bounce b uses set b & 1): the shadow launches of
+// bounce b may run while the extension launch of bounce b + 1, which clears
+// the set bounce b + 1 fills, is running.
+constexpr int kQRay0 = 0, kQRay1 = kXcds, kQShadow = 2 * kXcds;
+__host__ __device__ constexpr int qShadow(int parity, int light) { return kQShadow + (2 * parity + light) * kXcds; }
+// Work-chunk counters of the traversal launches (extend, and per parity the
+// HDRI and the light shadow), each on its own 128-byte line: a batch launch
+// draws from kChunkShares shares (share x first by the blocks b with
+// b % 64 == x), a single-scene launch from the first kXcds (one per XCD).
 constexpr int kChunkShares = 64;
-constexpr int kChunkStride = 32, kChunkExt = 0, kChunkShH = 1, kChunkShP = 2;
-constexpr int kQChunk = 4 * kXcds, kQCounters = kQChunk + 3 * kChunkShares * kChunkStride;
+constexpr int kChunkStride = 32, kChunkExt = 0;
+__host__ __device__ constexpr int chunkShadow(int parity, int light) { return 1 + 2 * parity + light; }
+constexpr int kQChunk = 6 * kXcds, kQCounters = kQChunk + 5 * kChunkShares * kChunkStride;
 constexpr uint32_t kChunkRays = 256;   // rays per work chunk (a multiple of 64)
 
 struct RenderArgs {
@@ -76,7 +81,8 @@ struct WfState {
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
     uint32_t* pathVis;      // recording renders: visibility bits per path slot (RenderArgs::recVis), else null
     uint32_t visBit;        // shadow launches of a recording render: the bit a visible query sets
-    uint32_t* counters;     // kQCounters: per-XCD segment counts of the four queues
+    uint32_t parity;        // bounce & 1: the shadow counter set of this bounce's launches
+    uint32_t* counters;     // kQCounters: per-XCD segment counts of the queues, work-chunk counters
     uint32_t segSize;       // entries per XCD segment of a compacted queue
     int* gstack;
     uint32_t gthreads;

@@ -38,7 +38,7 @@ namespace dev {
 #define EL_SLAB_AB 1
 #endif
 #ifndef EL_PUSH_BULK   // one stack-room check per node visit instead of one per push
-#define EL_PUSH_BULK 0
+#define EL_PUSH_BULK 1
 #endif
 
 // ---------------------------------------------------------------- vectors --
