@@ -63,6 +63,15 @@ extern "C" {
                                    device memory): el_adjoint then traces no
                                    ray.  The image is the same either way.   */
 
+#define EL_FLAG_SERIAL_SHADOWS 16u /* wavefront: run every shadow launch after
+                                     the bounce's shade on the one stream.  By
+                                     default the shadow launches of a bounce
+                                     overlap the next bounce's closest-hit
+                                     launch on a second stream when the launches
+                                     are small (generations under 16 M paths,
+                                     or from the third bounce on); the image is
+                                     the same either way                       */
+
 /* Passes, same numbering as the reference's `enum Passes` (kernel.h:7). */
 #define EL_PASS_BEAUTY     0
 #define EL_PASS_DENOISE    1  /* not produced (denoiser is out of scope)     */

@@ -310,3 +310,26 @@ def test_bvh8_renders_the_bvh4_image(gpu_ctx, scenes, mode):
     assert gpu_ctx.stats().bvh_width == 8
     assert np.array_equal(ca, cb) and ra == rb and va == vb
     assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", SCENES)
+def test_overlapped_shadows_equal_serial(gpu_ctx, scenes, kind):
+    """The shadow launches of a bounce overlapping the next extension on a
+    second stream (the default for small generations) leave the image, the
+    per-pixel counts and the tallies bit-identical to serial launches
+    (EL_FLAG_SERIAL_SHADOWS): the launches share no queue counters and the
+    radiance is summed in the same order.  Also with recording (the shadow
+    launches set visibility bits) and culling off."""
+    print(f"[render] overlapped vs serial shadows: {kind}", flush=True)
+    s = scenes[kind]
+    for extra in (0, eleven.FLAG_NO_CULL, eleven.FLAG_RECORD_PATHS):
+        out = []
+        for serial in (0, eleven.FLAG_SERIAL_SHADOWS):
+            gpu_ctx.upload(s)
+            st = gpu_ctx.render(16, depth=DEPTH, seed=5, flags=serial | extra)
+            img, cnt, rays, verts = gpu_ctx.read()
+            out.append((img, cnt, st.rays, st.vertices))
+        assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+        assert np.array_equal(out[0][1], out[1][1])
+        assert out[0][2:] == out[1][2:]
