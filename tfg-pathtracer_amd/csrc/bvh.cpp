@@ -17,8 +17,14 @@ namespace eleven {
 
 namespace {
 
-constexpr int kBins = 32;
-constexpr float kTraversalCost = 1.0f;   // relative to one triangle test
+#ifndef EL_SAH_BINS
+#define EL_SAH_BINS 32
+#endif
+#ifndef EL_SAH_NODE_COST
+#define EL_SAH_NODE_COST 1.0f
+#endif
+constexpr int kBins = EL_SAH_BINS;
+constexpr float kTraversalCost = EL_SAH_NODE_COST;   // relative to one triangle test
 constexpr int kParallelThreshold = 65536;
 
 struct Box {

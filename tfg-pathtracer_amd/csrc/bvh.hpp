@@ -57,7 +57,10 @@ struct alignas(16) QNode<8> {
 static_assert(sizeof(QNode<4>) == 64, "quantized BVH4 node must be 64 B");
 static_assert(sizeof(QNode<8>) == 96, "quantized BVH8 node must be 96 B");
 
-constexpr int kMaxLeafTris = 8;        // <= 16 fits the leaf encoding
+#ifndef EL_LEAF_MAX
+#define EL_LEAF_MAX 8
+#endif
+constexpr int kMaxLeafTris = EL_LEAF_MAX;   // <= 16 fits the leaf encoding
 constexpr int kLeafFirstLimit = 1 << 27;
 
 inline int32_t encodeLeaf(int first, int count) { return -(2 + ((first << 4) | (count - 1))); }
