@@ -941,11 +941,35 @@ __device__ __forceinline__ int visitNode(const uint4* __restrict__ nodes, int cu
     return visitWords<W, UNSORTED>(w, ray, tcull, st);
 }
 
-// Tests the triangles of leaf `code` (<= -2) and updates the closest hit:
-// smaller t wins, equal t goes to the lower original index.  For ANY the
-// first hit below r.t is taken and true is returned.
-//
-// The leaf's triangles are contiguous records; the three words of the next
+// Tests one triangle record and updates the closest hit: smaller t wins,
+// equal t goes to the lower original index.  For ANY the first hit below
+// r.t is taken and true is returned (the query ends).
+template <bool ANY>
+__device__ __forceinline__ bool hitTriangle(const float4& A, const float4& B, const float4& C, const TRay& ray,
+                                            TraceResult& r) {
+    float t, u, v;
+    if (mtHit(A, B, C, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, t, u, v)) {
+        const int id = __float_as_int(C.y);
+        if (ANY) {
+            if (t < r.t) {
+                r.tri = id;
+                r.t = t;
+                r.u = u;
+                r.v = v;
+                return true;
+            }
+        } else if (t < r.t || (t == r.t && (unsigned)id < (unsigned)r.tri)) {
+            r.tri = id;
+            r.t = t;
+            r.u = u;
+            r.v = v;
+        }
+    }
+    return false;
+}
+
+// Tests the triangles of leaf `code` (<= -2), as hitTriangle.  The leaf's
+// triangles are contiguous records; the three words of the next
 // triangle are loaded before the current one is tested, and all three
 // words of a triangle together (the vertex word is otherwise fetched only
 // after the determinant test), so a leaf costs about one memory round trip
@@ -971,24 +995,7 @@ __device__ __forceinline__ bool testLeaf(const float4* __restrict__ tris, int co
         float4 A = p[3 * i], B = p[3 * i + 1], C = p[3 * i + 2];
 #endif
         if (COUNT) cnt.tris++;
-        float t, u, v;
-        if (mtHit(A, B, C, ray.ox, ray.oy, ray.oz, ray.dx, ray.dy, ray.dz, t, u, v)) {
-            int id = __float_as_int(C.y);
-            if (ANY) {
-                if (t < r.t) {
-                    r.tri = id;
-                    r.t = t;
-                    r.u = u;
-                    r.v = v;
-                    return true;
-                }
-            } else if (t < r.t || (t == r.t && (unsigned)id < (unsigned)r.tri)) {
-                r.tri = id;
-                r.t = t;
-                r.u = u;
-                r.v = v;
-            }
-        }
+        if (hitTriangle<ANY>(A, B, C, ray, r)) return true;
     }
     return false;
 }
