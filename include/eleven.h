@@ -343,6 +343,11 @@ typedef struct el_batch el_batch;
 #define EL_BATCH_GRAPH 1u   /* capture the launch sequence in a hipGraph once,
                                replay it on later renders with equal plans  */
 #define EL_BATCH_NO_CULL 4u /* as EL_FLAG_NO_CULL                            */
+#define EL_BATCH_SERIAL_SHADOWS 8u /* as EL_FLAG_SERIAL_SHADOWS (batches overlap
+                                      a bounce's shadow launches with the next
+                                      extension from the third bounce on, or
+                                      always for generations under 16 M paths,
+                                      also inside a captured graph)           */
 
 typedef struct {
     uint32_t max_depth;    /* 0 -> 5                                        */

@@ -160,3 +160,22 @@ def test_batch_with_texture_maps_and_lobes_equals_standalone(gpu_ctx, scene_root
         assert np.mean(np.abs(img - ref) / (np.abs(ref) + 1e-3)) < 1e-5
         assert np.isfinite(img).all() and img[..., :3].max() > 0
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_overlapped_shadows_equal_serial(batch_scenes):
+    """Overlapping each bounce's shadow launches with the next extension on a
+    second stream (the default; here every generation is small) gives the
+    serial launch order's images and counts bit for bit, eager and replayed
+    from a captured graph whose fork and join edges come from the events."""
+    print("[batch] overlapped vs serial shadows, eager and graph", flush=True)
+    b = _batch(batch_scenes)
+    b.render(depth=DEPTH, queue_cap=4096, serial_shadows=True)
+    ref = [b.read(i) for i in range(len(batch_scenes))]
+    for graph in (False, True, True):
+        b.render(depth=DEPTH, queue_cap=4096, graph=graph)
+        for i in range(len(batch_scenes)):
+            img, cnt = b.read(i)
+            np.testing.assert_array_equal(cnt, ref[i][1])
+            np.testing.assert_array_equal(img.view(np.uint32), ref[i][0].view(np.uint32))
+    b.close()

@@ -1248,7 +1248,18 @@ struct el_batch {
     std::string graphKey;
     el_stats stats{};
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipStream_t shadowStream = nullptr;   // shadow launches of small bounces (as el_render)
+    std::vector<hipEvent_t> syncPool;
 };
+
+static hipEvent_t batchSyncEvent(el_batch* b, size_t i) {
+    while (b->syncPool.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        b->syncPool.push_back(e);
+    }
+    return b->syncPool[i];
+}
 
 static int bfail(el_batch* b, int code, const std::string& what) {
     if (b) b->err = what;
@@ -1281,6 +1292,7 @@ int el_batch_create(el_batch** out, int device) {
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) return EL_ERR_HIP;
     b->maxThreads = (uint32_t)prop.multiProcessorCount * 2048u;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
+    if (hipStreamCreateWithFlags(&b->shadowStream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
     if (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) return EL_ERR_HIP;
     if (b->dCounters.ensure(sizeof(unsigned long long) * kCounterWords) != hipSuccess) return EL_ERR_HIP;
     *out = b.release();
@@ -1295,7 +1307,10 @@ void el_batch_destroy(el_batch* b) {
     for (auto* c : b->scenes) el_destroy(c);
     if (b->e0) (void)hipEventDestroy(b->e0);
     if (b->e1) (void)hipEventDestroy(b->e1);
+    if (b->shadowStream) (void)hipStreamSynchronize(b->shadowStream);
+    for (auto e : b->syncPool) (void)hipEventDestroy(e);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->shadowStream) (void)hipStreamDestroy(b->shadowStream);
     delete b;
 }
 
@@ -1488,9 +1503,17 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
     unsigned long long* ctr = b->dCounters.as<unsigned long long>();
     EL_BHIP(b, hipMemsetAsync(ctr, 0, sizeof(unsigned long long) * kCounterWords, b->stream));
     const DevScene* dsc = b->dScenes.as<DevScene>();
+    // Shadow launches of small bounces overlap the next extension on a
+    // second stream, as in el_render (inside a captured graph the events
+    // become the graph's fork and join edges).
+    LaunchCfg lcs = lc;
+    lcs.stream = b->shadowStream;
+    const bool serialShadows = (p->flags & EL_BATCH_SERIAL_SHADOWS) != 0;
     auto launchAll = [&]() -> hipError_t {
         hipError_t e;
+        size_t ev = 0;
         for (const BatchGen& g : gens) {
+            hipEvent_t shadowsDone = nullptr;
             const BatchSeg* gseg = b->dSegs.as<BatchSeg>() + g.segOffset;
             if ((e = launchBtCamera(lc, dsc, gseg, g.nseg, g.count, depth, !(p->flags & EL_BATCH_NO_CULL), q)) !=
                 hipSuccess)
@@ -1504,18 +1527,37 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
                 uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
                 uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
                 q.parity = bo & 1u;
+                const bool overlap = !serialShadows && (g.count < kOverlapPaths || bo >= 2);
                 if ((e = launchBtExtend(lc, dsc, width, q, q.ray[cur], cntCur, cntNxt, g.count, ctr)) != hipSuccess)
                     return e;
+                if (shadowsDone && (e = hipStreamWaitEvent(b->stream, shadowsDone, 0)) != hipSuccess) return e;
                 if ((e = launchBtShade(lc, dsc, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, g.count, ctr)) !=
                     hipSuccess)
                     return e;
-                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowH, q.counters + qShadow(q.parity, 0), g.count,
-                                        ctr)) != hipSuccess)
-                    return e;
-                if ((e = launchBtShadow(lc, dsc, width, q, q.shadowP, q.counters + qShadow(q.parity, 1), g.count,
-                                        ctr)) != hipSuccess)
-                    return e;
+                shadowsDone = nullptr;
+                if (overlap) {
+                    hipEvent_t shaded = batchSyncEvent(b, ev++), done = batchSyncEvent(b, ev++);
+                    if (!shaded || !done) return hipErrorOutOfMemory;
+                    if ((e = hipEventRecord(shaded, b->stream)) != hipSuccess) return e;
+                    if ((e = hipStreamWaitEvent(b->shadowStream, shaded, 0)) != hipSuccess) return e;
+                    if ((e = launchBtShadow(lcs, dsc, width, q, q.shadowH, q.counters + qShadow(q.parity, 0),
+                                            g.count, ctr)) != hipSuccess)
+                        return e;
+                    if ((e = launchBtShadow(lcs, dsc, width, q, q.shadowP, q.counters + qShadow(q.parity, 1),
+                                            g.count, ctr)) != hipSuccess)
+                        return e;
+                    if ((e = hipEventRecord(done, b->shadowStream)) != hipSuccess) return e;
+                    shadowsDone = done;
+                } else {
+                    if ((e = launchBtShadow(lc, dsc, width, q, q.shadowH, q.counters + qShadow(q.parity, 0), g.count,
+                                            ctr)) != hipSuccess)
+                        return e;
+                    if ((e = launchBtShadow(lc, dsc, width, q, q.shadowP, q.counters + qShadow(q.parity, 1), g.count,
+                                            ctr)) != hipSuccess)
+                        return e;
+                }
             }
+            if (shadowsDone && (e = hipStreamWaitEvent(b->stream, shadowsDone, 0)) != hipSuccess) return e;
             if ((e = launchBtFinalize(lc, dsc, gseg, g.nseg, g.finTotal, q, ctr)) != hipSuccess) return e;
         }
         return hipSuccess;

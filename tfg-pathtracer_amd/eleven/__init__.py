@@ -497,7 +497,7 @@ class Context:
         return ms.value
 
 
-BATCH_GRAPH, BATCH_NO_CULL = 1, 4
+BATCH_GRAPH, BATCH_NO_CULL, BATCH_SERIAL_SHADOWS = 1, 4, 8
 
 
 class Batch:
@@ -521,8 +521,10 @@ class Batch:
         self.scenes.append(scene)
         return idx.value
 
-    def render(self, depth=5, block=0, queue_cap=0, bvh_width=0, refill=0, graph=False, cull=True) -> el_stats:
-        flags = (BATCH_GRAPH if graph else 0) | (0 if cull else BATCH_NO_CULL)
+    def render(self, depth=5, block=0, queue_cap=0, bvh_width=0, refill=0, graph=False, cull=True,
+               serial_shadows=False) -> el_stats:
+        flags = (BATCH_GRAPH if graph else 0) | (0 if cull else BATCH_NO_CULL) | \
+            (BATCH_SERIAL_SHADOWS if serial_shadows else 0)
         p = el_batch_params(depth, block, queue_cap, bvh_width, refill, flags)
         self._check(lib().el_batch_render(self._h, C.byref(p)))
         return self.stats()
