@@ -131,3 +131,43 @@ def test_duplicate_triangles_tie_to_lower_index(gpu_ctx):
     for i in np.nonzero(hit)[0][:200]:
         copies = np.nonzero(bid == bid[tri_g[i]])[0]
         assert tri_g[i] == copies.min()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("width", [4, 8])
+def test_tiny_distant_nodes_keep_every_subtree(gpu_ctx, width):
+    """A cluster of tiny triangles with a larger triangle behind it, seen
+    from 1000 units away on every axis (some 60 scene diagonals).  Node
+    boxes are padded to 2^-15 of the diagonal, so the cluster's nodes are
+    ~6e-4 across with grid steps ~4e-6: 255 steps are below the slab
+    test's widening at that distance, which cannot reject an empty slot's
+    inverted box there.  Such a slot must still never be entered, or a ray
+    through the cluster stops before it reaches the triangle behind."""
+    print(f"[trace] tiny cluster seen from far away, BVH{width}", flush=True)
+    rng = np.random.default_rng(21)
+    size = 3e-5
+    n = 96
+    c = rng.uniform(-size / 2, size / 2, size=(n, 1, 3))
+    tiny = c + rng.normal(scale=size / 40, size=(n, 3, 3))
+    # behind the cluster as seen from (-50, -50, -50): a triangle across (1, 1, 1)
+    u = np.array([1.0, -1.0, 0.0]) / np.sqrt(2.0)
+    w = np.array([1.0, 1.0, -2.0]) / np.sqrt(6.0)
+    back = np.array([10.0, 10.0, 10.0]) + 5.0 * np.stack([u, w, -(u + w)])
+    v = np.concatenate([tiny.reshape(n, 9), back.reshape(1, 9)]).astype(np.float32)
+    sc = eleven.SceneData.from_arrays(v, camera=dict(x_res=4, y_res=4))
+    gpu_ctx.upload(sc)
+    gpu_ctx.render(0, bvh_width=width)
+    m = 1 << 15
+    o = np.full((m, 3), -1000.0) + rng.uniform(-1e-2, 1e-2, size=(m, 3))
+    target = rng.uniform(-size, size, size=(m, 3))
+    rays = np.concatenate([o, target - o], -1).astype(np.float32)
+    rays[:, 3:] = raygen._norm(rays[:, 3:])
+    tri_g, t_g, _ = gpu_ctx.trace_closest(rays)
+    tb, tt, _, _ = orc.closest_brute(v, rays)
+    # the tiny triangles are below the intersection test's determinant
+    # threshold: every ray passes the cluster and hits the triangle behind
+    assert np.all(tb == n)
+    _check(tri_g, t_g, tb, tt, f"tiny cluster BVH{width}")
+    # any hit within 2000 (the triangle behind is ~1750 away): every ray
+    occ = gpu_ctx.trace_any(rays, np.full(m, 2000.0, np.float32))
+    assert np.all(tt < 2000.0) and np.all(occ)

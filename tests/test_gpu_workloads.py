@@ -133,7 +133,7 @@ def test_c1_crops_match_oracle_common_random_numbers(gpu_ctx, c1_scene, c1_refbv
         print(f"  window {w}: counts equal {np.mean(gc == oc):.4f}, pixels within 1e-3 {frac:.4f}, "
               f"mean rel {rel.mean():.2e}", flush=True)
         assert np.mean(gc == oc) > 0.99
-        assert frac > 0.97 and rel.mean() < 2e-3
+        assert frac > 0.995 and rel.mean() < 5e-4
 
 
 # ------------------------------------------------------------------- batch --
@@ -181,8 +181,11 @@ def test_c2_scenes_match_oracle_common_random_numbers(gpu_ctx, c2_work, i):
     om = _crop(o / np.maximum(oc, 1)[..., None], win, H)
     g = _crop(img[..., :3].astype(np.float64), win, H)
     rel = np.abs(g - om) / (np.abs(om) + 1e-3)
+    frac = np.mean(rel.max(-1) < 1e-3)
+    print(f"  counts equal {np.mean(_crop(cnt, win, H) == _crop(oc, win, H)):.4f}, pixels within 1e-3 {frac:.4f}, "
+          f"mean rel {rel.mean():.2e}", flush=True)
     assert np.mean(_crop(cnt, win, H) == _crop(oc, win, H)) > 0.99
-    assert np.mean(rel.max(-1) < 1e-3) > 0.97 and rel.mean() < 2e-3
+    assert frac > 0.97 and rel.mean() < 2e-3
 
 
 # ----------------------------------------------------------------- adjoint --

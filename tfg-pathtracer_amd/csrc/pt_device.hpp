@@ -839,7 +839,12 @@ __device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, floa
     // A + E with E = 2^-19 |A| of their own axis (an axis whose direction is
     // near zero has a huge A, but also huge distances of its own), and the
     // 2^-20 |tf| widening below covers the |t| term: the test stays
-    // conservative.
+    // conservative.  An empty slot (child -1) has an inverted box, 255 grid
+    // steps from near to far plane; where 255 steps are below the rounding
+    // of A (a node smaller than ~4e-6 of its distance from the origin on
+    // every axis) the box passes the widened test, so the slot's code is
+    // checked as well: entering it would end the traversal (-1 is the empty
+    // stack's code).
     const float Ax = (px - ray.ox) * ray.ix, Ay = (py - ray.oy) * ray.iy, Az = (pz - ray.oz) * ray.iz;
     const float Bx = sx * ray.ix, By = sy * ray.iy, Bz = sz * ray.iz;
     const float Ex = fabsf(Ax) * (1.0f / 524288.0f), Ey = fabsf(Ay) * (1.0f / 524288.0f),
@@ -857,7 +862,7 @@ __device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, floa
         const float tn = fmaxf(fmaxf(tx0, ty0), fmaxf(tz0, 0.0f));
         float tf = fminf(fminf(tx1, ty1), fminf(tz1, tcull));
         tf = fmaf(fabsf(tf), 1.0f / 1048576.0f, tf);
-        keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+        keys[i] = (tn <= tf && child[i] != -1) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
     }
 #else
 #pragma unroll
@@ -873,7 +878,7 @@ __device__ __forceinline__ int visitWords(const uint4* np, const TRay& ray, floa
                                fmaxf((bz0 - ray.oz) * ray.iz, 0.0f));
         float tf = fminf(fminf((bx1 - ray.ox) * ray.ix, (by1 - ray.oy) * ray.iy), fminf((bz1 - ray.oz) * ray.iz, tcull));
         tf = fmaf(fabsf(tf), 1.0f / 1048576.0f, tf);
-        keys[i] = (tn <= tf) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
+        keys[i] = (tn <= tf && child[i] != -1) ? ((__float_as_uint(tn) & ~7u) | (uint32_t)i) : 0xFFFFFFFFu;
     }
 #endif
     if constexpr (UNSORTED) {
