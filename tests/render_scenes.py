@@ -95,3 +95,22 @@ def maps_scene(res=(64, 48), seed=7, mirror_roughness=0.1):
     return eleven.SceneData.from_arrays(v, cam, normals=n, uvs=uv, tangents=tg, object_id=ob,
                                         mesh_material=np.arange(len(mats)), materials=mats, textures=textures,
                                         hdri=scenegen.sky_hdr(64, 32, peak=40.0, sigma_deg=8.0), lights=lights)
+
+
+def deep_stack_scene(n=20000, res=(64, 64), seed=31):
+    """A cloud of large, heavily overlapping triangles around the camera: most
+    rays enter many sibling boxes at every level, so traversal stacks grow
+    past the LDS part and spill to the global area (the tree's stack bound
+    is far above 16).  Face normals, one diffuse material, a sky gradient and
+    one point light, so both shadow launches of every bounce have rays."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-3.0, 3.0, size=(n, 1, 3))
+    v = c + rng.normal(scale=0.8, size=(n, 3, 3))
+    fn = np.cross(v[:, 1] - v[:, 0], v[:, 2] - v[:, 0])
+    fn /= np.maximum(np.linalg.norm(fn, axis=-1, keepdims=True), 1e-12)
+    normals = np.repeat(fn[:, None, :], 3, axis=1)
+    sky = np.zeros((16, 32, 3), np.float32)
+    sky[...] = np.linspace(1.5, 0.2, 16, dtype=np.float32)[:, None, None]
+    return eleven.SceneData.from_arrays(v.astype(np.float32).reshape(n, 9), camera=dict(x_res=res[0], y_res=res[1]),
+                                        normals=normals.astype(np.float32).reshape(n, 9), hdri=sky,
+                                        lights=[((0.5, 4.0, 0.5), (20.0, 20.0, 20.0))])

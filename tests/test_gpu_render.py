@@ -21,7 +21,7 @@ import pytest
 
 import eleven
 import oracle_py as orc
-from render_scenes import DEPTH, SCENES, make_scene
+from render_scenes import DEPTH, SCENES, deep_stack_scene, make_scene
 from pathlib import Path
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
@@ -310,6 +310,27 @@ def test_bvh8_renders_the_bvh4_image(gpu_ctx, scenes, mode):
     assert gpu_ctx.stats().bvh_width == 8
     assert np.array_equal(ca, cb) and ra == rb and va == vb
     assert np.array_equal(a, b)
+
+
+@pytest.mark.gpu
+def test_overlapped_shadows_equal_serial_with_spilled_stacks(gpu_ctx):
+    """As below on a scene whose traversal stacks spill to the global area:
+    the overlapped shadow launches spill to their own area, not the one the
+    extension running beside them uses (both index it by grid thread)."""
+    print("[render] overlapped vs serial shadows, spilling stacks", flush=True)
+    s = deep_stack_scene()
+    gpu_ctx.upload(s)
+    out = []
+    for serial in (0, eleven.FLAG_SERIAL_SHADOWS, 0):
+        gpu_ctx.clear()
+        st = gpu_ctx.render(16, depth=DEPTH, seed=9, flags=serial)
+        img, cnt, _, _ = gpu_ctx.read()
+        out.append((img, cnt, st.rays, st.vertices, st.stack_bound))
+    assert out[0][4] > 20, out[0][4]   # stacks deeper than the 16 LDS entries (28)
+    for o in out[1:]:
+        assert np.array_equal(out[0][0].view(np.uint32), o[0].view(np.uint32))
+        assert np.array_equal(out[0][1], o[1])
+        assert out[0][2:4] == o[2:4]
 
 
 @pytest.mark.gpu

@@ -318,7 +318,7 @@ static int uploadBvh(el_ctx* c, int width) {
     EL_HIP(c, upload(c->dNodes, c->wide.nodes.data(), c->wide.nodes.size(), c->stream));
     EL_HIP(c, upload(c->dTris, c->wide.triStream.data(), c->wide.triStream.size() * sizeof(float), c->stream));
     c->gcap = std::max(1, c->wide.stackBound + 2 - dev::kShortStack);
-    EL_HIP(c, c->dGstack.ensure((size_t)c->gcap * c->maxThreads * sizeof(int)));
+    EL_HIP(c, c->dGstack.ensure((size_t)kSpillSets * c->gcap * c->maxThreads * sizeof(int)));
     c->ds.nodes = c->dNodes.p;
     c->ds.tris = c->dTris.as<float4>();
     c->ds.width = width;
@@ -791,13 +791,15 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                 }
                 hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(z0, ls.stream));
-                if (recordHits) q.visBit = 1u << (2 * b);
-                EL_HIP(c, launchWfShadow(ls, c->ds, q, q.shadowH, q.counters + qShadow(q.parity, 0), a.genCount,
+                // beside the extension: the shadow stream's spill area
+                WfState qs = q;
+                if (overlap) qs.gstack = q.gstack + (size_t)c->gcap * lc.maxThreads;
+                if (recordHits) qs.visBit = 1u << (2 * b);
+                EL_HIP(c, launchWfShadow(ls, c->ds, qs, q.shadowH, q.counters + qShadow(q.parity, 0), a.genCount,
                                          ctr));
-                if (recordHits) q.visBit = 1u << (2 * b + 1);
-                EL_HIP(c, launchWfShadow(ls, c->ds, q, q.shadowP, q.counters + qShadow(q.parity, 1), a.genCount,
+                if (recordHits) qs.visBit = 1u << (2 * b + 1);
+                EL_HIP(c, launchWfShadow(ls, c->ds, qs, q.shadowP, q.counters + qShadow(q.parity, 1), a.genCount,
                                          ctr));
-                q.visBit = 0;
                 EL_HIP(c, hipEventRecord(z1, ls.stream));
                 shadowsDone = nullptr;
                 if (overlap) {
@@ -1424,7 +1426,7 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
         total += work[i];
     }
     EL_BHIP(b, upload(b->dScenes, recs.data(), S * sizeof(DevScene), b->stream));
-    EL_BHIP(b, b->dGstack.ensure((size_t)gcap * b->maxThreads * sizeof(int)));
+    EL_BHIP(b, b->dGstack.ensure((size_t)kSpillSets * gcap * b->maxThreads * sizeof(int)));
     // generation plan
     uint64_t Q = p->queue_cap;
     if (!Q) {
@@ -1540,10 +1542,12 @@ static int batchRender(el_batch* b, const el_batch_params* p, uint32_t sppCap, d
                     if (!shaded || !done) return hipErrorOutOfMemory;
                     if ((e = hipEventRecord(shaded, b->stream)) != hipSuccess) return e;
                     if ((e = hipStreamWaitEvent(b->shadowStream, shaded, 0)) != hipSuccess) return e;
-                    if ((e = launchBtShadow(lcs, dsc, width, q, q.shadowH, q.counters + qShadow(q.parity, 0),
+                    WfState qs = q;   // the shadow stream's spill area
+                    qs.gstack = q.gstack + (size_t)gcap * lc.maxThreads;
+                    if ((e = launchBtShadow(lcs, dsc, width, qs, q.shadowH, q.counters + qShadow(q.parity, 0),
                                             g.count, ctr)) != hipSuccess)
                         return e;
-                    if ((e = launchBtShadow(lcs, dsc, width, q, q.shadowP, q.counters + qShadow(q.parity, 1),
+                    if ((e = launchBtShadow(lcs, dsc, width, qs, q.shadowP, q.counters + qShadow(q.parity, 1),
                                             g.count, ctr)) != hipSuccess)
                         return e;
                     if ((e = hipEventRecord(done, b->shadowStream)) != hipSuccess) return e;

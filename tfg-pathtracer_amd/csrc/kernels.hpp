@@ -98,6 +98,13 @@ struct BatchSeg {
     uint32_t finStart, pad;
 };
 
+// Global traversal-stack spill areas per context (gcap x maxThreads ints
+// each): launches on the main stream use the first, the shadow launches
+// that run beside the next extension on the shadow stream the second (a
+// spill slot is addressed by the thread's grid index, which two kernels
+// running at once share).
+constexpr int kSpillSets = 2;
+
 struct LaunchCfg {
     hipStream_t stream;
     uint32_t block;         // traversal kernels (trace, extend, shadow) and the plain megakernel
