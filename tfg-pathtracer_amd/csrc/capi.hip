@@ -124,7 +124,7 @@ struct el_ctx {
 
     DevBuf dNodes, dTris, dShade, dMats, dTexs, dTexels, dLights, dEnvCdf, dEnvGuide, dEnvMap, dEnvBad, dEnvTab, dTexels8, dTexLut, dTangents;
     DevBuf dGstack, dAccum, dCounts, dAov, dCounters;
-    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters;
+    DevBuf wRay0, wRay1, wHits, wShH, wShP, wT, wL, wPix, wSample, wQuiet, wAov, wCounters, wPend;
     size_t wfCap = 0, wfQcap = 0;
 
     DevBuf tRays, tTmax, tTri, tT, tUV, tOcc;   // scratch for host-pointer ray queries
@@ -149,9 +149,10 @@ struct el_ctx {
 
     std::vector<hipEvent_t> evPool;
     // Wavefront renders: the shadow launches of bounce b run on `shadowStream`
-    // while the extension launch of bounce b + 1 runs on `stream`; syncPool
-    // holds the events that order them (no timing).
-    hipStream_t shadowStream = nullptr;
+    // (HDRI) and `shadowStreamP` (point lights) while the extension launch of
+    // bounce b + 1 runs on `stream`; syncPool holds the events that order
+    // them (no timing).
+    hipStream_t shadowStream = nullptr, shadowStreamP = nullptr;
     std::vector<hipEvent_t> syncPool;
 };
 
@@ -214,6 +215,7 @@ int el_create(el_ctx** out, int device) {
     c->maxThreads = (uint32_t)c->cus * 2048u;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->shadowStream, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c->shadowStreamP, hipStreamNonBlocking) != hipSuccess) return EL_ERR_HIP;
     if (c->dCounters.ensure(sizeof(unsigned long long) * kCounterWords) != hipSuccess) return EL_ERR_HIP;
     if (scratchAddress(device, &c->scratch) != hipSuccess) return EL_ERR_HIP;
     *out = c.release();
@@ -225,10 +227,12 @@ void el_destroy(el_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->shadowStream) (void)hipStreamSynchronize(c->shadowStream);
+    if (c->shadowStreamP) (void)hipStreamSynchronize(c->shadowStreamP);
     for (auto e : c->evPool) (void)hipEventDestroy(e);
     for (auto e : c->syncPool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->shadowStream) (void)hipStreamDestroy(c->shadowStream);
+    if (c->shadowStreamP) (void)hipStreamDestroy(c->shadowStreamP);
     delete c;
 }
 
@@ -587,6 +591,7 @@ static int ensureWf(el_ctx* c, size_t cap, size_t qcap, bool aov) {
     EL_HIP(c, c->wPix.ensure(cap * 4));
     EL_HIP(c, c->wSample.ensure(cap * 4));
     EL_HIP(c, c->wQuiet.ensure(cap * 8));
+    EL_HIP(c, c->wPend.ensure(cap * 16));
     EL_HIP(c, c->wCounters.ensure(kQCounters * sizeof(uint32_t)));
     c->wfCap = std::max(c->wfCap, cap);
     c->wfQcap = std::max(c->wfQcap, qcap);
@@ -606,7 +611,7 @@ static int wfPlan(el_ctx* c, const LaunchCfg& lc, uint64_t total, uint64_t npad,
     if (!Q) {
         size_t freeB = 0, totalB = 0;
         EL_HIP(c, hipMemGetInfo(&freeB, &totalB));
-        const uint64_t perPath = 6 * 32 + 44 + (aov ? 48 : 0);
+        const uint64_t perPath = 6 * 32 + 60 + (aov ? 48 : 0);
         Q = std::max<uint64_t>(1u << 20, freeB / 4 / perPath);
     }
     if (Q >= npad)
@@ -625,7 +630,7 @@ static int wfPlan(el_ctx* c, const LaunchCfg& lc, uint64_t total, uint64_t npad,
     // the launches index Q path slots and qcap queue entries in every buffer
     if (c->wRay0.n < qcap * 32 || c->wRay1.n < qcap * 32 || c->wHits.n < qcap * 16 || c->wShH.n < qcap * 48 ||
         c->wShP.n < qcap * 48 || c->wT.n < Q * 16 || c->wL.n < Q * 16 || c->wPix.n < Q * 4 || c->wSample.n < Q * 4 ||
-        c->wQuiet.n < Q * 8 || (aov && c->wAov.n < Q * 48))
+        c->wQuiet.n < Q * 8 || c->wPend.n < Q * 16 || (aov && c->wAov.n < Q * 48))
         return fail(c, EL_ERR_STATE, "wavefront buffers smaller than the generation (internal error)");
     q.ray[0] = c->wRay0.as<float4>();
     q.ray[1] = c->wRay1.as<float4>();
@@ -638,6 +643,7 @@ static int wfPlan(el_ctx* c, const LaunchCfg& lc, uint64_t total, uint64_t npad,
     q.pathSample = c->wSample.as<uint32_t>();
     q.pathQuiet = c->wQuiet.as<uint64_t>();
     q.pathAov = aov ? c->wAov.as<float4>() : nullptr;
+    q.pendP = c->wPend.as<float4>();
     q.counters = c->wCounters.as<uint32_t>();
     q.segSize = (uint32_t)segSize;
     q.gstack = lc.gstack;
@@ -738,6 +744,9 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
         if (recordHits) {
             EL_HIP(c, c->wVis.ensure(Q * sizeof(uint32_t)));
             q.pathVis = c->wVis.as<uint32_t>();
+            // the shadow launches set visibility bits of one word per path:
+            // they stay in order, and the point-light terms are added at once
+            q.pendP = nullptr;
         }
         a.npad = (uint32_t)npad;
         size_t ev = 2;
@@ -748,14 +757,20 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
             a.genCount = (uint32_t)std::min<uint64_t>(Q, total - start);
             EL_HIP(c, launchWfCamera(lc, c->ds, a, q));
             // Per bounce b: extend (stream) -> shade (stream) -> both shadow
-            // launches (shadowStream).  The shadows of bounce b need only
-            // what shade b wrote and update nothing the extension of bounce
-            // b + 1 reads (their counters are set b & 1, the extension
-            // clears set (b + 1) & 1), so the two overlap; shade b + 1 and
-            // finalize wait for them (they add to the path radiance).
-            LaunchCfg lcs = lc;
+            // launches (shadowStream, shadowStreamP).  The shadows of bounce
+            // b need only what shade b wrote and update nothing the extension
+            // of bounce b + 1 reads (their counters are set b & 1, the
+            // extension clears set (b + 1) & 1), so the three overlap; shade
+            // b + 1 and finalize wait for them (they add to the path
+            // radiance).  The point-light launch writes its terms to pendP,
+            // which shade b + 1 or finalize adds after the HDRI term, so it
+            // also runs beside the HDRI launch (a recording render keeps the
+            // two in order on one stream).
+            LaunchCfg lcs = lc, lcp = lc;
             lcs.stream = c->shadowStream;
-            hipEvent_t shadowsDone = nullptr;
+            lcp.stream = c->shadowStreamP;
+            const size_t spillSet = (size_t)c->gcap * lc.maxThreads;
+            hipEvent_t doneH = nullptr, doneP = nullptr;
             // Large launches keep the GPU busy on their own, and overlapping
             // two of them costs cache hits (the extension's L2 hit rate fell
             // from 0.70 to 0.62 next to the first bounce's shadows); small
@@ -766,6 +781,9 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                 const bool overlap = !serial && (a.genCount < kOverlapPaths || b >= 2);
                 a.bounce = b;
                 q.parity = b & 1u;
+                // point-light terms deferred where the two shadow launches run
+                // at once (elsewhere the P launch adds them itself)
+                q.deferP = overlap && q.pendP != nullptr ? 1u : 0u;
                 const int cur = b & 1, nxt = cur ^ 1;
                 uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
                 uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
@@ -775,44 +793,61 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                 EL_HIP(c, hipEventRecord(x1, c->stream));
                 timed.push_back({ev, 0});
                 ev += 2;
-                if (shadowsDone) EL_HIP(c, hipStreamWaitEvent(c->stream, shadowsDone, 0));
+                if (doneH) EL_HIP(c, hipStreamWaitEvent(c->stream, doneH, 0));
+                if (doneP) EL_HIP(c, hipStreamWaitEvent(c->stream, doneP, 0));
                 hipEvent_t y0 = poolEvent(c, ev), y1 = poolEvent(c, ev + 1);
                 EL_HIP(c, hipEventRecord(y0, c->stream));
                 EL_HIP(c, launchWfShade(lc, c->ds, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, a.genCount, ctr));
                 EL_HIP(c, hipEventRecord(y1, c->stream));
                 timed.push_back({ev, 1});
                 ev += 2;
-                const LaunchCfg& ls = overlap ? lcs : lc;
+                // beside the extension, each shadow launch spills to its own area
+                const bool split = q.deferP != 0u;   // H and P on their own streams
+                const LaunchCfg& lsH = overlap ? lcs : lc;
+                const LaunchCfg& lsP = split ? lcp : lsH;
+                WfState qh = q, qp = q;
+                if (overlap) qh.gstack = q.gstack + spillSet;
+                qp.gstack = split ? q.gstack + 2 * spillSet : qh.gstack;
+                if (recordHits) {
+                    qh.visBit = 1u << (2 * b);
+                    qp.visBit = 1u << (2 * b + 1);
+                }
                 if (overlap) {
-                    hipEvent_t shaded = syncEvent(c, 2 * b);
+                    hipEvent_t shaded = syncEvent(c, 3 * b);
                     if (!shaded) return fail(c, EL_ERR_HIP, "event creation failed");
                     EL_HIP(c, hipEventRecord(shaded, c->stream));
                     EL_HIP(c, hipStreamWaitEvent(c->shadowStream, shaded, 0));
+                    if (split) EL_HIP(c, hipStreamWaitEvent(c->shadowStreamP, shaded, 0));
                 }
                 hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
-                EL_HIP(c, hipEventRecord(z0, ls.stream));
-                // beside the extension: the shadow stream's spill area
-                WfState qs = q;
-                if (overlap) qs.gstack = q.gstack + (size_t)c->gcap * lc.maxThreads;
-                if (recordHits) qs.visBit = 1u << (2 * b);
-                EL_HIP(c, launchWfShadow(ls, c->ds, qs, q.shadowH, q.counters + qShadow(q.parity, 0), a.genCount,
+                EL_HIP(c, hipEventRecord(z0, lsH.stream));
+                EL_HIP(c, launchWfShadow(lsH, c->ds, qh, q.shadowH, q.counters + qShadow(q.parity, 0), a.genCount,
                                          ctr));
-                if (recordHits) qs.visBit = 1u << (2 * b + 1);
-                EL_HIP(c, launchWfShadow(ls, c->ds, qs, q.shadowP, q.counters + qShadow(q.parity, 1), a.genCount,
-                                         ctr));
-                EL_HIP(c, hipEventRecord(z1, ls.stream));
-                shadowsDone = nullptr;
-                if (overlap) {
-                    hipEvent_t done = syncEvent(c, 2 * b + 1);
-                    if (!done) return fail(c, EL_ERR_HIP, "event creation failed");
-                    EL_HIP(c, hipEventRecord(done, c->shadowStream));
-                    shadowsDone = done;
-                }
+                EL_HIP(c, hipEventRecord(z1, lsH.stream));
                 timed.push_back({ev, 2});
                 ev += 2;
+                hipEvent_t w0 = poolEvent(c, ev), w1 = poolEvent(c, ev + 1);
+                EL_HIP(c, hipEventRecord(w0, lsP.stream));
+                EL_HIP(c, launchWfShadow(lsP, c->ds, qp, q.shadowP, q.counters + qShadow(q.parity, 1), a.genCount,
+                                         ctr));
+                EL_HIP(c, hipEventRecord(w1, lsP.stream));
+                timed.push_back({ev, 2});
+                ev += 2;
+                doneH = doneP = nullptr;
+                if (overlap) {
+                    doneH = syncEvent(c, 3 * b + 1);
+                    if (!doneH) return fail(c, EL_ERR_HIP, "event creation failed");
+                    EL_HIP(c, hipEventRecord(doneH, c->shadowStream));
+                }
+                if (split) {
+                    doneP = syncEvent(c, 3 * b + 2);
+                    if (!doneP) return fail(c, EL_ERR_HIP, "event creation failed");
+                    EL_HIP(c, hipEventRecord(doneP, c->shadowStreamP));
+                }
                 extendLaunches++;
             }
-            if (shadowsDone) EL_HIP(c, hipStreamWaitEvent(c->stream, shadowsDone, 0));
+            if (doneH) EL_HIP(c, hipStreamWaitEvent(c->stream, doneH, 0));
+            if (doneP) EL_HIP(c, hipStreamWaitEvent(c->stream, doneP, 0));
             EL_HIP(c, launchWfFinalize(lc, c->ds, a, q, c->dAccum.as<float4>(), c->dCounts.as<uint32_t>(), aovBuf,
                                        ctr));
         }

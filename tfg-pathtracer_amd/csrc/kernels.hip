@@ -851,15 +851,30 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_extend(P sc, WfState q, const float4* __restric
 // starts at zero there and is not stored before); later bounces skip the
 // read-modify-write when every component is zero, which leaves L bit for bit
 // as the addition would (x + 0 == x, x + -0 == x; a NaN is still added).
-__device__ __forceinline__ void addRadiance(float4* __restrict__ pathL, uint32_t slot, int bounce, v3 c) {
+// `pre` (later bounces): the previous bounce's deferred point-light term,
+// added first, where its shadow launch used to add it.  It is zero when the
+// light was blocked (a visible term is never zero: its query is traced only
+// then).
+__device__ __forceinline__ bool isZero4(float4 a) { return a.x == 0.f && a.y == 0.f && a.z == 0.f; }
+__device__ __forceinline__ void addRadiance(float4* __restrict__ pathL, uint32_t slot, int bounce, v3 c,
+                                            float4 pre = make_float4(0.f, 0.f, 0.f, 0.f)) {
     if (bounce == 0) {
         pathL[slot] = make_float4(0.f + c.x, 0.f + c.y, 0.f + c.z, 1.f);   // 0 + -0 == +0, as a sum
         return;
     }
-    if (c.x == 0.f && c.y == 0.f && c.z == 0.f) return;
-    const float4 L = pathL[slot];
-    pathL[slot] = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+    const bool addPre = !isZero4(pre), addC = !(c.x == 0.f && c.y == 0.f && c.z == 0.f);
+    if (!addPre && !addC) return;
+    float4 L = pathL[slot];
+    if (addPre) L = make_float4(L.x + pre.x, L.y + pre.y, L.z + pre.z, L.w);
+    if (addC) L = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, L.w);
+    pathL[slot] = L;
 }
+
+// Marks on pathL.w / pathT.w that a path's last shade emitted a point-light
+// query whose term is deferred (pendP): 2 on the radiance of a path that
+// ends there (finalize adds the term), 1 on the throughput of one that
+// continues (the next shade adds it).
+constexpr float kPendEnd = 2.f, kPendNext = 1.f;
 
 // Escaped rays (a texture lookup) and hits (full shading) arrive mixed in
 // the queue.  Each block classifies a chunk of blockDim entries, adds the
@@ -903,7 +918,8 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                     const DevScene& s = sc.get(__float_as_uint(B.w));
                     const float4 Tf = bounce == 0 ? make_float4(1, 1, 1, 0) : q.pathT[slot];
                     const v3 ev = envLookup(s, mk(B.x, B.y, B.z)) * mk(Tf.x, Tf.y, Tf.z);
-                    addRadiance(q.pathL, slot, bounce, ev);
+                    addRadiance(q.pathL, slot, bounce, ev,
+                                Tf.w == kPendNext ? q.pendP[slot] : make_float4(0.f, 0.f, 0.f, 0.f));
                 }
                 hit = tri >= 0;
                 if constexpr (!P::kMany) {
@@ -970,7 +986,8 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
             uint4 rb = rng.block(2 + bounce);
             uint32_t r[4] = {rb.x, rb.y, rb.z, rb.w};
             vo = shadeVertex(s, d, mk(Tf.x, Tf.y, Tf.z), sh, hd, r);
-            addRadiance(q.pathL, slot, bounce, vo.base);
+            addRadiance(q.pathL, slot, bounce, vo.base,
+                        Tf.w == kPendNext ? q.pendP[slot] : make_float4(0.f, 0.f, 0.f, 0.f));
             if (bounce == 0 && q.pathAov) {
                 v3 at, ab;
                 surfaceTangents(s, sh, at, ab);
@@ -995,7 +1012,9 @@ EL_KERNEL(EL_WPE_SHADE) void k_wf_shade(P sc, RenderArgs a, WfState q, const flo
                                 (front && dot(hd.normal, vo.ldir) > -1e-6f));
             cont = !last && !(a.cull && isZero(vo.newT) && (q.pathQuiet[slot] >> (bounce + 1)) == 0ull);
             // only an extended path reads its throughput again
-            if (cont) q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, 0);
+            const bool pend = q.deferP != 0u && wantP;
+            if (cont) q.pathT[slot] = make_float4(vo.newT.x, vo.newT.y, vo.newT.z, pend ? kPendNext : 0.f);
+            if (pend && !cont) reinterpret_cast<float*>(q.pathL)[4 * (uint64_t)slot + 3] = kPendEnd;
         }
         const bool want[3] = {cont, wantH, wantP};
         uint64_t at[3];
@@ -1058,10 +1077,18 @@ EL_KERNEL(EL_WPE_TRAV) void k_wf_shadow(P sc, WfState q, const float4* __restric
         }
         return true;
     };
+    // point-light queries of a render with deferred terms: every query
+    // writes its term (zero if blocked) for the next shade or finalize
+    const bool defer = !REC && !P::kMany && q.deferP != 0u && sh == q.shadowP;
     auto store = [&](uint32_t i, const TraceResult& h, bool) {
-        if (h.tri >= 0) return;
+        if (h.tri >= 0 && !defer) return;
         const uint64_t e = qm.entry(i, segSize);
         const uint32_t slot = __float_as_uint(sh[3 * e].w);
+        if (defer) {
+            const float4 C = sh[3 * e + 2];
+            q.pendP[slot] = h.tri >= 0 ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(C.x, C.y, C.z, 0.f);
+            return;
+        }
         const float4 C = sh[3 * e + 2];
         if constexpr (REC) {
             q.pathVis[slot] |= q.visBit;
@@ -1096,7 +1123,11 @@ EL_KERNEL(1) void k_wf_finalize(DevScene s, RenderArgs a, WfState q, float4* __r
         if (aov)
             for (int k = 0; k < 3; k++) A[k] = aov[3 * (uint64_t)idx + k];
         for (uint32_t slot = j; slot < a.genCount; slot += a.npad, sample++) {
-            const float4 Lf = q.pathL[slot];
+            float4 Lf = q.pathL[slot];
+            if (Lf.w == kPendEnd) {   // the last bounce's deferred point-light term, added last
+                const float4 Pp = q.pendP[slot];
+                if (!isZero4(Pp)) Lf = make_float4(Lf.x + Pp.x, Lf.y + Pp.y, Lf.z + Pp.z, Lf.w);
+            }
             paths++;
             if (a.rec) a.rec[(uint64_t)sample * s.cam.xRes * s.cam.yRes + idx] = make_float4(Lf.x, Lf.y, Lf.z, 1.f);
             if (a.recVis) a.recVis[(uint64_t)sample * s.cam.xRes * s.cam.yRes + idx] = q.pathVis[slot];

@@ -80,6 +80,10 @@ struct WfState {
     uint64_t* pathQuiet;    // per path slot: bounces whose HDRI number is NaN-prone (bit b, b < 64)
     float4* pathAov;        // 3 per path slot (normal, tangent, bitangent) or null
     uint32_t* pathVis;      // recording renders: visibility bits per path slot (RenderArgs::recVis), else null
+    float4* pendP;          // per path slot: the point-light term of the last shadow P launch (0: blocked), or
+                            // null; with deferP the launch writes it here instead of adding it to pathL, so
+                            // it can run beside the HDRI shadow launch, and shade / finalize add it in order
+    uint32_t deferP;        // this bounce's point-light terms go to pendP (shade marks, shadow P writes)
     uint32_t visBit;        // shadow launches of a recording render: the bit a visible query sets
     uint32_t parity;        // bounce & 1: the shadow counter set of this bounce's launches
     uint32_t* counters;     // kQCounters: per-XCD segment counts of the queues, work-chunk counters
@@ -99,11 +103,11 @@ struct BatchSeg {
 };
 
 // Global traversal-stack spill areas per context (gcap x maxThreads ints
-// each): launches on the main stream use the first, the shadow launches
-// that run beside the next extension on the shadow stream the second (a
-// spill slot is addressed by the thread's grid index, which two kernels
-// running at once share).
-constexpr int kSpillSets = 2;
+// each): launches on the main stream use the first; the shadow launches
+// that run beside the next extension use the second (HDRI) and third (point
+// light).  A spill slot is addressed by the thread's grid index, which
+// kernels running at once share.
+constexpr int kSpillSets = 3;
 
 struct LaunchCfg {
     hipStream_t stream;
