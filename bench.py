@@ -592,7 +592,9 @@ def main():
     t0 = time.perf_counter()
     for k in range(args.steps):
         ctx.clear()
-        st = ctx.render(args.spp, seed=0, **kw)
+        # per-launch HIP events on each launch's stream (the roofline's
+        # average extension time; ~0.1 ms of a 100 ms frame)
+        st = ctx.render(args.spp, seed=0, flags=eleven.FLAG_KERNEL_TIMES, **kw)
         traced += st.rays
         ext_ms += st.extend_ms
         ext_launches += st.extend_launches

@@ -72,6 +72,14 @@ extern "C" {
                                      or from the third bounce on); the image is
                                      the same either way                       */
 
+#define EL_FLAG_KERNEL_TIMES 32u /* wavefront: time every closest-hit, shade
+                                     and shadow launch with events on its
+                                     stream (el_stats extend_ms, shade_ms,
+                                     shadow_ms; 0 without the flag).  The
+                                     events add a few microseconds between
+                                     launches: a 128x128 x 32 spp render takes
+                                     ~10 % longer with them                   */
+
 /* Passes, same numbering as the reference's `enum Passes` (kernel.h:7). */
 #define EL_PASS_BEAUTY     0
 #define EL_PASS_DENOISE    1  /* not produced (denoiser is out of scope)     */
@@ -155,7 +163,9 @@ typedef struct {
     uint64_t vertices;         /* shaded path vertices (reference kPaths)    */
     uint64_t paths;            /* paths traced                               */
     double   render_ms;        /* wall time of the last el_render (device)   */
-    double   extend_ms;        /* summed closest-hit kernel time (wavefront) */
+    double   extend_ms;        /* summed closest-hit kernel time (wavefront,
+                                  EL_FLAG_KERNEL_TIMES; shadow_ms, shade_ms
+                                  likewise)                                  */
     double   shadow_ms;        /* summed shadow kernel time (wavefront); el_render
                                   runs a bounce's shadow launches on a second
                                   stream while the next bounce's closest-hit

@@ -354,3 +354,20 @@ def test_overlapped_shadows_equal_serial(gpu_ctx, scenes, kind):
         assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
         assert np.array_equal(out[0][1], out[1][1])
         assert out[0][2:] == out[1][2:]
+
+
+@pytest.mark.gpu
+def test_kernel_times_only_on_request(gpu_ctx, scenes):
+    """Per-launch timing (EL_FLAG_KERNEL_TIMES) fills the per-kind times and
+    leaves the image bit for bit as it is without it."""
+    print("[render] per-launch kernel times on request", flush=True)
+    s = scenes["spheres"]
+    gpu_ctx.upload(s)
+    out = []
+    for flags in (0, eleven.FLAG_KERNEL_TIMES):
+        gpu_ctx.clear()
+        st = gpu_ctx.render(8, depth=DEPTH, seed=3, flags=flags)
+        out.append((gpu_ctx.read()[0], st.extend_ms, st.shade_ms, st.shadow_ms, st.render_ms))
+    assert out[0][1] == 0.0 and out[0][2] == 0.0 and out[0][3] == 0.0 and out[0][4] > 0.0
+    assert out[1][1] > 0.0 and out[1][2] > 0.0 and out[1][3] > 0.0
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))

@@ -777,6 +777,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
             // ones last one ray's chain of dependent fetches, which the
             // overlap hides.
             const bool serial = (p->flags & EL_FLAG_SERIAL_SHADOWS) != 0;
+            const bool kernelTimes = (p->flags & EL_FLAG_KERNEL_TIMES) != 0;
             for (uint32_t b = 0; b < depth; b++) {
                 const bool overlap = !serial && (a.genCount < kOverlapPaths || b >= 2);
                 a.bounce = b;
@@ -788,18 +789,18 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                 uint32_t* cntCur = q.counters + (cur ? kQRay1 : kQRay0);
                 uint32_t* cntNxt = q.counters + (nxt ? kQRay1 : kQRay0);
                 hipEvent_t x0 = poolEvent(c, ev), x1 = poolEvent(c, ev + 1);
-                EL_HIP(c, hipEventRecord(x0, c->stream));
+                if (kernelTimes) EL_HIP(c, hipEventRecord(x0, c->stream));
                 EL_HIP(c, launchWfExtend(lc, c->ds, q, q.ray[cur], cntCur, cntNxt, a.genCount, countTrav, ctr));
-                EL_HIP(c, hipEventRecord(x1, c->stream));
-                timed.push_back({ev, 0});
+                if (kernelTimes) EL_HIP(c, hipEventRecord(x1, c->stream));
+                if (kernelTimes) timed.push_back({ev, 0});
                 ev += 2;
                 if (doneH) EL_HIP(c, hipStreamWaitEvent(c->stream, doneH, 0));
                 if (doneP) EL_HIP(c, hipStreamWaitEvent(c->stream, doneP, 0));
                 hipEvent_t y0 = poolEvent(c, ev), y1 = poolEvent(c, ev + 1);
-                EL_HIP(c, hipEventRecord(y0, c->stream));
+                if (kernelTimes) EL_HIP(c, hipEventRecord(y0, c->stream));
                 EL_HIP(c, launchWfShade(lc, c->ds, a, q, q.ray[cur], cntCur, q.ray[nxt], cntNxt, a.genCount, ctr));
-                EL_HIP(c, hipEventRecord(y1, c->stream));
-                timed.push_back({ev, 1});
+                if (kernelTimes) EL_HIP(c, hipEventRecord(y1, c->stream));
+                if (kernelTimes) timed.push_back({ev, 1});
                 ev += 2;
                 // beside the extension, each shadow launch spills to its own area
                 const bool split = q.deferP != 0u;   // H and P on their own streams
@@ -820,18 +821,18 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                     if (split) EL_HIP(c, hipStreamWaitEvent(c->shadowStreamP, shaded, 0));
                 }
                 hipEvent_t z0 = poolEvent(c, ev), z1 = poolEvent(c, ev + 1);
-                EL_HIP(c, hipEventRecord(z0, lsH.stream));
+                if (kernelTimes) EL_HIP(c, hipEventRecord(z0, lsH.stream));
                 EL_HIP(c, launchWfShadow(lsH, c->ds, qh, q.shadowH, q.counters + qShadow(q.parity, 0), a.genCount,
                                          ctr));
-                EL_HIP(c, hipEventRecord(z1, lsH.stream));
-                timed.push_back({ev, 2});
+                if (kernelTimes) EL_HIP(c, hipEventRecord(z1, lsH.stream));
+                if (kernelTimes) timed.push_back({ev, 2});
                 ev += 2;
                 hipEvent_t w0 = poolEvent(c, ev), w1 = poolEvent(c, ev + 1);
-                EL_HIP(c, hipEventRecord(w0, lsP.stream));
+                if (kernelTimes) EL_HIP(c, hipEventRecord(w0, lsP.stream));
                 EL_HIP(c, launchWfShadow(lsP, c->ds, qp, q.shadowP, q.counters + qShadow(q.parity, 1), a.genCount,
                                          ctr));
-                EL_HIP(c, hipEventRecord(w1, lsP.stream));
-                timed.push_back({ev, 2});
+                if (kernelTimes) EL_HIP(c, hipEventRecord(w1, lsP.stream));
+                if (kernelTimes) timed.push_back({ev, 2});
                 ev += 2;
                 doneH = doneP = nullptr;
                 if (overlap) {

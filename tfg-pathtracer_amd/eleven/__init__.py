@@ -21,6 +21,7 @@ LIB_PATH = PKG_DIR / "libeleven.so"
 EL_OK, EL_ERR_ARG, EL_ERR_IO, EL_ERR_HIP, EL_ERR_STATE, EL_ERR_RANGE, EL_ERR_NOMEM = 0, -1, -2, -3, -4, -5, -6
 MODE_PLAIN, MODE_WAVEFRONT = 0, 1
 FLAG_AOV, FLAG_COUNT_TRAVERSAL, FLAG_NO_CULL, FLAG_RECORD_PATHS, FLAG_SERIAL_SHADOWS = 1, 2, 4, 8, 16
+FLAG_KERNEL_TIMES = 32   # per-launch event timing (el_stats extend_ms, shade_ms, shadow_ms)
 ADJ_ACCUMULATE, ADJ_DEVICE_RESIDUAL = 1, 2
 PASS_BEAUTY, PASS_DENOISE, PASS_NORMAL, PASS_TANGENT, PASS_BITANGENT = 0, 1, 2, 3, 4
 
