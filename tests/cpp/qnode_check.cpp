@@ -29,7 +29,7 @@ template <int W> int run(std::mt19937& g) {
         for (int a = 0; a < 3; a++) {
             float s = std::ldexp(1.0f, (int)((q.exps >> (8 * a)) & 0xff) - 127);
             for (int i = 0; i < W; i++) {
-                uint32_t ql = (q.q[a*2][i/4] >> (8*(i%4))) & 0xff, qh = (q.q[a*2+1][i/4] >> (8*(i%4))) & 0xff;
+                uint32_t ql = q.plane(a*2, i), qh = q.plane(a*2+1, i);
                 float bl = std::fma((float)ql, s, q.p[a]), bh = std::fma((float)qh, s, q.p[a]);
                 double el = (double)q.p[a] + (double)ql * s, eh = (double)q.p[a] + (double)qh * s;
                 if ((double)bl != el || (double)bh != eh) { bad++; if (bad < 5) printf("inexact\n"); }

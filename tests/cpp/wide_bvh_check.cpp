@@ -41,8 +41,8 @@ static void walk(const WideBvh& w, const std::vector<float>& v, int node, std::v
         Box b;
         for (int a = 0; a < 3; a++) {
             const float s = std::ldexp(1.0f, (int)((n.exps >> (8 * a)) & 0xff) - 127);
-            const uint32_t ql = (n.q[a * 2][i / 4] >> (8 * (i % 4))) & 0xff;
-            const uint32_t qh = (n.q[a * 2 + 1][i / 4] >> (8 * (i % 4))) & 0xff;
+            const uint32_t ql = n.plane(a * 2, i);
+            const uint32_t qh = n.plane(a * 2 + 1, i);
             b.lo[a] = std::fma((float)ql, s, n.p[a]);
             b.hi[a] = std::fma((float)qh, s, n.p[a]);
         }

@@ -349,8 +349,8 @@ bool quantizeNode(const WideNode<W>& in, QNode<W>& out) {
     }
     for (int a = 0; a < 3; a++)
         for (int i = 0; i < W; i++) {
-            out.q[a * 2 + 0][i / 4] |= (uint32_t)qlo[a][i] << (8 * (i % 4));
-            out.q[a * 2 + 1][i / 4] |= (uint32_t)qhi[a][i] << (8 * (i % 4));
+            out.setPlane(a * 2 + 0, i, qlo[a][i]);
+            out.setPlane(a * 2 + 1, i, qhi[a][i]);
         }
     for (int i = 0; i < W; i++) out.child[i] = in.child[i];
     return true;

@@ -46,15 +46,19 @@ struct alignas(16) QNode<4> {
     uint32_t q[6][1];
     int32_t child[4];
     int32_t pad[2];
+    uint32_t plane(int as, int i) const { return (q[as][0] >> (8 * i)) & 0xffu; }
+    void setPlane(int as, int i, uint32_t v) { q[as][0] |= v << (8 * i); }
 };
+static_assert(sizeof(QNode<4>) == 64, "quantized BVH4 node must be 64 B");
 template <>
 struct alignas(16) QNode<8> {
     float p[3];
     uint32_t exps;
     uint32_t q[6][2];
     int32_t child[8];
+    uint32_t plane(int as, int i) const { return (q[as][i / 4] >> (8 * (i % 4))) & 0xffu; }
+    void setPlane(int as, int i, uint32_t v) { q[as][i / 4] |= v << (8 * (i % 4)); }
 };
-static_assert(sizeof(QNode<4>) == 64, "quantized BVH4 node must be 64 B");
 static_assert(sizeof(QNode<8>) == 96, "quantized BVH8 node must be 96 B");
 
 #ifndef EL_LEAF_MAX
