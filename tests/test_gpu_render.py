@@ -371,3 +371,30 @@ def test_kernel_times_only_on_request(gpu_ctx, scenes):
     assert out[0][1] == 0.0 and out[0][2] == 0.0 and out[0][3] == 0.0 and out[0][4] > 0.0
     assert out[1][1] > 0.0 and out[1][2] > 0.0 and out[1][3] > 0.0
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("depth", [1, 2, 3])
+def test_deferred_light_terms_at_shallow_depths(gpu_ctx, scenes, depth):
+    """Point-light terms of overlapped bounces wait in a per-path buffer for
+    the next shade or, on a path's last bounce, for finalize; at depths 1-3
+    most or all paths end with such a term pending.  Images, counts and
+    tallies equal the serial launch order (the launch adds the term) bit for
+    bit; the plain megakernel's counts exactly and its image as in
+    test_plain_equals_wavefront (its FP32 shading rounds a few values
+    differently, 1-2 ulp)."""
+    print(f"[render] deferred point-light terms, depth {depth}", flush=True)
+    s = scenes["terrain_bokeh"]   # 4 point lights: every hit draws a light query
+    gpu_ctx.upload(s)
+    out = []
+    for mode, flags in ((eleven.MODE_WAVEFRONT, 0), (eleven.MODE_WAVEFRONT, eleven.FLAG_SERIAL_SHADOWS),
+                        (eleven.MODE_PLAIN, 0)):
+        gpu_ctx.clear()
+        st = gpu_ctx.render(8, depth=depth, seed=11, mode=mode, flags=flags)
+        img, cnt, _, _ = gpu_ctx.read()
+        out.append((img, cnt, st.rays, st.vertices))
+    assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2:] == out[1][2:]
+    assert np.array_equal(out[0][1], out[2][1])
+    np.testing.assert_allclose(out[0][0], out[2][0], rtol=1e-4, atol=1e-6)

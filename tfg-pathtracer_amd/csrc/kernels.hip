@@ -335,10 +335,12 @@ __device__ __forceinline__ bool quietDeath(const DevScene& s, const PathRng& rng
 // Bit b set: bounce b's environment random number falls in a bucket that can
 // reach a pdf-0 / pdf-inf pixel (see quietDeath).  A path that dies at
 // bounce k < maxDepth - 1 stops quietly iff (mask >> (k + 1)) == 0; maxDepth
-// <= 64, so every bounce has its bit and the shift stays below 64.
+// <= 64, so every bounce has its bit and the shift stays below 64.  Bit 0
+// is never read (a death is checked from the next bounce on): no Philox
+// block is drawn for it.
 __device__ __forceinline__ uint64_t nanProneBounces(const DevScene& s, const PathRng& rng, int maxDepth) {
     uint64_t m = 0;
-    for (int b = 0; b < maxDepth && b < 64; b++) {
+    for (int b = 1; b < maxDepth && b < 64; b++) {
         const uint32_t bucket = rng.block(2 + b).w >> 16;
         m |= (uint64_t)((s.envBad[bucket >> 5] >> (bucket & 31)) & 1u) << b;
     }
