@@ -179,3 +179,24 @@ def test_batch_overlapped_shadows_equal_serial(batch_scenes):
             np.testing.assert_array_equal(cnt, ref[i][1])
             np.testing.assert_array_equal(img.view(np.uint32), ref[i][0].view(np.uint32))
     b.close()
+
+
+@pytest.mark.gpu
+def test_batch_overlapped_shadows_with_spilled_stacks(batch_scenes):
+    """A batch holding a scene whose traversal stacks spill past the 16 LDS
+    entries (stack bound 28): the shadow launches beside an extension spill
+    to their own area, so overlapped and serial launch orders agree bit for
+    bit, eager and replayed from the captured graph."""
+    from render_scenes import deep_stack_scene
+    print("[batch] overlapped vs serial shadows with spilling stacks", flush=True)
+    scenes = [(deep_stack_scene(), 8, 41)] + list(batch_scenes[:2])
+    b = _batch(scenes)
+    b.render(depth=DEPTH, serial_shadows=True)
+    ref = [b.read(i) for i in range(len(scenes))]
+    for graph in (False, True, True):
+        b.render(depth=DEPTH, graph=graph)
+        for i in range(len(scenes)):
+            img, cnt = b.read(i)
+            np.testing.assert_array_equal(cnt, ref[i][1])
+            np.testing.assert_array_equal(img.view(np.uint32), ref[i][0].view(np.uint32))
+    b.close()
