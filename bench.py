@@ -48,16 +48,16 @@ def ensure_built():
         __graft_entry__.build()
 
 
-def measured_traffic(kernel="k_wf_extend"):
-    """HBM bytes per launch of `kernel` on this workload at the default
-    settings, from the committed FETCH_SIZE / WRITE_SIZE counter summary
-    (rocprofv3 --pmc, scaled by a 1 GiB copy).  None if absent."""
+def measured_traffic(kernel="k_wf_extend", per="launch"):
+    """HBM bytes per launch (or per frame) of `kernel` on this workload at the
+    default settings, from the committed FETCH_SIZE / WRITE_SIZE counter
+    summary (rocprofv3 --pmc, scaled by a 1 GiB copy).  None if absent."""
     f = ROOT / "profiles" / "hbm_traffic_heightfield_1080p_64spp.md"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
         parts = line.split()
-        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+        if len(parts) == 3 and parts[0] == f"traffic_bytes_per_{per}" and parts[1] == kernel:
             return int(parts[2])
     return None
 
@@ -665,6 +665,22 @@ def main():
         out["config"]["extend_ms_per_step"] = round(ext_ms / args.steps, 2)
         out["config"]["shade_ms_per_step"] = round(st.shade_ms, 2)
         out["config"]["shadow_ms_per_step"] = round(st.shadow_ms, 2)
+        # the shading kernel (memory-bound): the bytes each queue entry needs
+        # -- a hit: its ray 32 + hit record 16 + throughput 16 + shading record
+        # 128 + HDRI sample entry 32 + texel 4 + radiance read/write 32, and
+        # writes the next ray 32 + throughput 16 + two shadow entries 96; an
+        # escape: ray 32 + hit 16 + throughput 16 + environment texel 12 +
+        # radiance 32 -- against the counted traffic of the same frame
+        hits, escapes = st.vertices, max(0, st.extend_rays - st.vertices)
+        shade_algo = hits * 404 + escapes * 108
+        shade_traffic = measured_traffic("k_wf_shade", "frame") if defaults and width == 4 else None
+        out["shade"] = {"kernel": "k_wf_shade", "ms_per_step": round(st.shade_ms, 3),
+                        "hits": hits, "escapes": escapes, "algo_bytes_per_step": int(shade_algo),
+                        "achieved_gbs": round(shade_algo / (st.shade_ms * 1e-3) / 1e9, 1),
+                        "frac_of_copy": round(shade_algo / (st.shade_ms * 1e-3) / 1e9 / copy_gbs, 3)
+                        if copy_gbs else None,
+                        "traffic_per_step": shade_traffic,
+                        "traffic_over_algorithmic": round(shade_traffic / shade_algo, 3) if shade_traffic else None}
 
     if not args.no_plain and mode == eleven.MODE_WAVEFRONT:
         # the reference's structure: one thread per pixel, every path traced to the end
