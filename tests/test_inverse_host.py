@@ -46,3 +46,20 @@ def test_survey_views_aim_at_their_cells():
         hit = np.array([cam.position.x, cam.position.y, cam.position.z]) + t * d
         np.testing.assert_allclose(hit, (xa, 0.0, za), atol=1e-4)
         assert cam.x_res == 512 and cam.y_res == 512
+
+
+def test_param_materials_equal_with_params():
+    """The reused material array, rewritten in place, holds byte for byte
+    the structs with_params builds (both round the parameters to float32)."""
+    import ctypes as C
+    from eleven.autograd import ParamMaterials, with_params
+    base = [eleven.make_material() for _ in range(5)]
+    for i, m in enumerate(base):
+        m.clearcoat, m.sheen, m.eta = 0.1 * i, 0.05 * i, 1.1 + 0.1 * i
+    pm = ParamMaterials(base)
+    rng = np.random.default_rng(3)
+    for _ in range(4):
+        p = rng.random((len(base), 6)) * 1.7 - 0.2
+        a = pm.set(p)
+        b = (eleven.el_material * len(base))(*with_params(base, p))
+        assert C.string_at(C.addressof(a), C.sizeof(a)) == C.string_at(C.addressof(b), C.sizeof(b))

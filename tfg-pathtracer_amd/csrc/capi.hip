@@ -726,6 +726,7 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     uint64_t extendLaunches = 0;
     EL_HIP(c, hipEventRecord(e0, c->stream));
     const bool countTrav = (p->flags & 2u) != 0;
+    unsigned long long h[kCounterWords];   // the launches' tallies, read back with the last sync
 
     if (p->mode == EL_MODE_PLAIN) {
         for (uint32_t s = 0; s < p->spp; s++) {
@@ -853,7 +854,8 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
                                        ctr));
         }
         EL_HIP(c, hipEventRecord(e1, c->stream));
-        EL_HIP(c, hipEventSynchronize(e1));
+        EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        EL_HIP(c, hipStreamSynchronize(c->stream));   // the tallies and every event
         for (auto& t : timed) {
             float ms = 0;
             EL_HIP(c, hipEventElapsedTime(&ms, c->evPool[t.first], c->evPool[t.first + 1]));
@@ -862,13 +864,11 @@ extern "C" int el_render(el_ctx* c, const el_render_params* p) {
     }
     if (p->mode == EL_MODE_PLAIN) {
         EL_HIP(c, hipEventRecord(e1, c->stream));
-        EL_HIP(c, hipEventSynchronize(e1));
+        EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+        EL_HIP(c, hipStreamSynchronize(c->stream));
     }
     float totalMs = 0;
     EL_HIP(c, hipEventElapsedTime(&totalMs, e0, e1));
-    unsigned long long h[kCounterWords];
-    EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    EL_HIP(c, hipStreamSynchronize(c->stream));
     if (c->sampleBase == 0) {
         c->accSeed = p->seed;
         c->accDepth = depth;
@@ -990,6 +990,10 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
     EL_HIP(c, hipMemcpyAsync(hm.data(), c->aMat.p, M * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     unsigned long long h[kCounterWords];
     EL_HIP(c, hipMemcpyAsync(h, ctr, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    // the texel gradients with the same sync
+    if (ntex && dtex)
+        EL_HIP(c, hipMemcpyAsync(dtex, c->aTex.p, ntex * sizeof(float),
+                                 devp ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c->stream));
     EL_HIP(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     EL_HIP(c, hipEventElapsedTime(&ms, e0, e1));
@@ -1001,8 +1005,6 @@ extern "C" int el_adjoint(el_ctx* c, const el_adjoint_params* p, const float* di
         else
             std::memcpy(dmat, fm.data(), fm.size() * sizeof(float));
     }
-    if (ntex && dtex)
-        EL_HIP(c, hipMemcpy(dtex, c->aTex.p, ntex * sizeof(float), devp ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost));
     c->adjSumValid = true;
     c->adjSumTex = gtex;
     c->stats.adjoint_ms = ms;

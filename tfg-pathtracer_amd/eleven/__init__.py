@@ -400,7 +400,8 @@ class Context:
         return s
 
     def set_materials(self, mats):
-        arr = (el_material * len(mats))(*mats)
+        """mats: el_material structs (a list, or a ctypes array passed as is)."""
+        arr = mats if isinstance(mats, C.Array) else (el_material * len(mats))(*mats)
         self._check(lib().el_set_materials(self._h, arr, len(mats)))
 
     def residual_l2(self, target) -> float:

@@ -52,6 +52,23 @@ def with_params(base: list, params: np.ndarray) -> list:
     return out
 
 
+class ParamMaterials:
+    """A ctypes array of `base`'s materials whose six parameters are
+    rewritten in place (through a numpy view of the array) for each call:
+    the values with_params gives, without building the structs again."""
+
+    def __init__(self, base: list):
+        self.array = (el_material * len(base))(*base)
+        self._view = np.ctypeslib.as_array(self.array)
+
+    def set(self, params: np.ndarray):
+        p = np.asarray(params, np.float64).astype(np.float32)   # as the c_float fields round
+        v = self._view
+        v["albedo"]["x"], v["albedo"]["y"], v["albedo"]["z"] = p[:, 0], p[:, 1], p[:, 2]
+        v["roughness"], v["metallic"], v["specular"] = p[:, 3], p[:, 4], p[:, 5]
+        return self.array
+
+
 class Renderer:
     """Differentiable forward render of one uploaded scene."""
 
@@ -61,11 +78,12 @@ class Renderer:
         self.spp, self.depth, self.seed = spp, depth, seed
         self.grad_texture, self.mode, self.record = grad_texture, mode, record
         self.base = scene.materials()
+        self._mats = ParamMaterials(self.base)
 
     def render(self, params: np.ndarray, texture: np.ndarray | None = None, read: bool = True) -> np.ndarray | None:
         """Forward only (numpy in, numpy out; read=False leaves the image on
         the device, e.g. for Context.residual_l2)."""
-        self.ctx.set_materials(with_params(self.base, params))
+        self.ctx.set_materials(self._mats.set(params))
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
         self.ctx.clear()

@@ -31,7 +31,7 @@ import numpy as np
 import torch
 
 from . import FLAG_RECORD_PATHS, MODE_WAVEFRONT, Context, SceneData, el_camera, make_camera
-from .autograd import material_params, with_params
+from .autograd import ParamMaterials, material_params
 
 # seed offset of the render whose paths the adjoint replays (decorrelation)
 DECORRELATE_SEED = 0x5EED0001
@@ -88,9 +88,10 @@ class MultiView:
         self.spp, self.depth, self.grad_texture, self.mode = spp, depth, grad_texture, mode
         self.record = record
         self.base = scene.materials()
+        self._mats = ParamMaterials(self.base)
 
     def set_params(self, params: np.ndarray, texture: np.ndarray | None = None):
-        self.ctx.set_materials(with_params(self.base, params))
+        self.ctx.set_materials(self._mats.set(params))
         if texture is not None:
             self.ctx.set_texture(self.grad_texture, np.ascontiguousarray(texture, np.float32))
 
