@@ -143,21 +143,27 @@ class InverseResult:
 
 
 def block_up(c: np.ndarray, k: int) -> np.ndarray:
-    """Texture of k x k texel blocks from its block values c [h][w][3]."""
-    return c if k == 1 else np.repeat(np.repeat(c, k, axis=0), k, axis=1)
+    """Texture of k x k texel blocks from its block values c [h][w][3] (one
+    broadcast copy, on torch's threads)."""
+    if k == 1:
+        return c
+    h, w, ch = c.shape
+    src = torch.from_numpy(np.ascontiguousarray(c))
+    return src[:, None, :, None, :].expand(h, k, w, k, ch).reshape(h * k, w * k, ch).numpy()
 
 
 def block_sum(t: np.ndarray, k: int) -> np.ndarray:
-    """Adjoint of block_up: the sum over each k x k block."""
+    """Adjoint of block_up: the sum over each k x k block (rows of a block
+    first, then its columns: two passes of contiguous row additions)."""
     if k == 1:
         return t
     h, w, ch = t.shape
-    return t.reshape(h // k, k, w // k, k, ch).sum(axis=(1, 3))
+    rows = np.ascontiguousarray(t).reshape(h // k, k, w * ch).sum(axis=1)
+    return rows.reshape(h // k, w // k, k, ch).sum(axis=2)
 
 
 def block_mean(t: np.ndarray, k: int) -> np.ndarray:
-    h, w, ch = t.shape
-    return t if k == 1 else t.reshape(h // k, k, w // k, k, ch).mean(axis=(1, 3))
+    return t if k == 1 else block_sum(t, k) / t.dtype.type(k * k)
 
 
 def optimize(mv: MultiView, params0, texture0, targets, steps: int = 200, lr: float = 1e-2, seed: int = 1000,
@@ -208,7 +214,7 @@ def optimize(mv: MultiView, params0, texture0, targets, steps: int = 200, lr: fl
         opt_p.step()
         if t is not None:
             opt_t.zero_grad()
-            t.grad = torch.from_numpy(block_sum(dtex.astype(np.float32), kcur))
+            t.grad = torch.from_numpy(block_sum(np.asarray(dtex, np.float32), kcur))
             opt_t.step()
         with torch.no_grad():
             p.copy_(torch.max(torch.min(p, hi), lo))
