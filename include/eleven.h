@@ -65,11 +65,12 @@ extern "C" {
 
 #define EL_FLAG_SERIAL_SHADOWS 16u /* wavefront: run every shadow launch after
                                      the bounce's shade on the one stream.  By
-                                     default the shadow launches of a bounce
-                                     overlap the next bounce's closest-hit
-                                     launch on a second stream when the launches
-                                     are small (generations under 16 M paths,
-                                     or from the third bounce on); the image is
+                                     default the two shadow launches of a
+                                     bounce run on two more streams, beside
+                                     each other and the next bounce's
+                                     closest-hit launch, when the launches are
+                                     small (generations under 16 M paths, or
+                                     from the third bounce on); the image is
                                      the same either way                       */
 
 #define EL_FLAG_KERNEL_TIMES 32u /* wavefront: time every closest-hit, shade
@@ -167,10 +168,10 @@ typedef struct {
                                   EL_FLAG_KERNEL_TIMES; shadow_ms, shade_ms
                                   likewise)                                  */
     double   shadow_ms;        /* summed shadow kernel time (wavefront); el_render
-                                  runs a bounce's shadow launches on a second
-                                  stream while the next bounce's closest-hit
-                                  launch runs, so the three times overlap and
-                                  their sum exceeds render_ms                */
+                                  runs a bounce's two shadow launches on two
+                                  more streams while the next bounce's
+                                  closest-hit launch runs, so the times
+                                  overlap and their sum exceeds render_ms    */
     double   shade_ms;         /* summed shading kernel time (wavefront)     */
     uint64_t extend_launches;  /* number of closest-hit launches timed       */
     uint64_t extend_rays;      /* rays processed by those launches           */
