@@ -3,6 +3,7 @@
 // a fixed depth of 18 with 14 bins and no leaf-size termination, which makes
 // every ray descend 18 levels regardless of scene size.
 #include "bvh.hpp"
+#include "host_parallel.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -245,11 +246,12 @@ void packWide(const Bvh2& b, const float pad, WideBvh& out) {
         q.pop_front();
         maxDepth = std::max(maxDepth, it.depth);
         // gather up to W children by repeatedly opening the largest internal child
-        std::vector<int> kids = {b.nodes[it.bvh2].left, b.nodes[it.bvh2].right};
-        while ((int)kids.size() < W) {
+        int kids[W] = {b.nodes[it.bvh2].left, b.nodes[it.bvh2].right};
+        int nk = 2;
+        while (nk < W) {
             int best = -1;
             float bestA = -1;
-            for (int k = 0; k < (int)kids.size(); k++) {
+            for (int k = 0; k < nk; k++) {
                 const Bvh2Node& c = b.nodes[kids[k]];
                 if (c.leaf()) continue;
                 Box bx;
@@ -263,9 +265,8 @@ void packWide(const Bvh2& b, const float pad, WideBvh& out) {
             if (best < 0) break;
             int open = kids[best];
             kids[best] = b.nodes[open].left;
-            kids.push_back(b.nodes[open].right);
+            kids[nk++] = b.nodes[open].right;
         }
-        int nk = (int)kids.size();
         int sd = it.stackDepth + (nk - 1);
         maxStack = std::max(maxStack, sd);
         WideNode<W> node = emptyNode();
@@ -292,8 +293,12 @@ void packWide(const Bvh2& b, const float pad, WideBvh& out) {
     out.nodeCount = wn.size();
     out.nodeStride = sizeof(QNode<W>);
     std::vector<QNode<W>> qn(wn.size());
-    for (size_t i = 0; i < wn.size(); i++)
-        if (!quantizeNode<W>(wn[i], qn[i])) throw std::runtime_error("non-finite bounding box (vertex data)");
+    std::atomic<bool> finite{true};
+    parallelFor(wn.size(), [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++)
+            if (!quantizeNode<W>(wn[i], qn[i])) finite = false;
+    });
+    if (!finite) throw std::runtime_error("non-finite bounding box (vertex data)");
     out.nodes.resize(qn.size() * sizeof(QNode<W>));
     std::memcpy(out.nodes.data(), qn.data(), out.nodes.size());
     out.stackBound = std::max(1, maxStack);
@@ -390,22 +395,24 @@ WideBvh collapseWide(const Bvh2& b, const float* v, int n, int width) {
         packWide<8>(b, pad, out);
     out.order = b.order;
     out.triStream.assign((size_t)n * 12, 0.f);
-    for (int i = 0; i < n; i++) {
-        int o = b.order[i];
-        const float* t = v + (size_t)o * 9;
-        float* s = &out.triStream[(size_t)i * 12];
-        s[0] = t[0];
-        s[1] = t[1];
-        s[2] = t[2];
-        s[3] = t[3] - t[0];
-        s[4] = t[4] - t[1];
-        s[5] = t[5] - t[2];
-        s[6] = t[6] - t[0];
-        s[7] = t[7] - t[1];
-        s[8] = t[8] - t[2];
-        int32_t oi = o;
-        std::memcpy(&s[9], &oi, 4);
-    }
+    parallelFor((size_t)n, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++) {
+            int o = b.order[i];
+            const float* t = v + (size_t)o * 9;
+            float* s = &out.triStream[i * 12];
+            s[0] = t[0];
+            s[1] = t[1];
+            s[2] = t[2];
+            s[3] = t[3] - t[0];
+            s[4] = t[4] - t[1];
+            s[5] = t[5] - t[2];
+            s[6] = t[6] - t[0];
+            s[7] = t[7] - t[1];
+            s[8] = t[8] - t[2];
+            int32_t oi = o;
+            std::memcpy(&s[9], &oi, 4);
+        }
+    });
     return out;
 }
 

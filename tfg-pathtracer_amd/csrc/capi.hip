@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -22,6 +23,7 @@
 #include "../../include/eleven.h"
 #include "bvh.hpp"
 #include "hdri_table.hpp"
+#include "host_parallel.hpp"
 #include "kernels.hpp"
 #include "pt_device.hpp"
 #include "scene.hpp"
@@ -246,17 +248,23 @@ const char* el_last_error(const el_ctx* c) { return c ? c->err.c_str() : "null c
 // not in the table.  The table is non-decreasing (the loader's fastPow of
 // i / 255), so each value is found by binary search.
 static bool packLdr(const float* rgb, size_t n, const float (&table)[256], uint32_t* out) {
-    for (size_t i = 0; i < n; i++) {
-        uint32_t w = 0;
-        for (int ch = 0; ch < 3; ch++) {
-            const float v = rgb[3 * i + ch];
-            const float* p = std::lower_bound(table, table + 256, v);
-            if (p == table + 256 || std::memcmp(p, &v, 4) != 0) return false;
-            w |= (uint32_t)(p - table) << (8 * ch);
+    std::atomic<bool> ok{true};
+    parallelFor(n, [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1 && ok.load(std::memory_order_relaxed); i++) {
+            uint32_t w = 0;
+            for (int ch = 0; ch < 3; ch++) {
+                const float v = rgb[3 * i + ch];
+                const float* p = std::lower_bound(table, table + 256, v);
+                if (p == table + 256 || std::memcmp(p, &v, 4) != 0) {
+                    ok = false;
+                    return;
+                }
+                w |= (uint32_t)(p - table) << (8 * ch);
+            }
+            out[i] = w;
         }
-        out[i] = w;
-    }
-    return true;
+    });
+    return ok;
 }
 
 // The device camera: the host fields plus the six trig values the reference
@@ -393,18 +401,20 @@ int el_upload_scene(el_ctx* c, const el_scene_view* v) {
     // shading records and corner tangents, indexed by original triangle
     std::vector<float> shade((size_t)v->tri_count * dev::kShadeFloats, 0.f);
     std::vector<float> tans((size_t)v->tri_count * dev::kTanFloats, 0.f);
-    for (uint32_t i = 0; i < v->tri_count; i++) {
-        float* S = &shade[(size_t)i * dev::kShadeFloats];
-        std::memcpy(S, v->vertices + (size_t)i * 9, 9 * sizeof(float));
-        if (v->normals) std::memcpy(S + 9, v->normals + (size_t)i * 9, 9 * sizeof(float));
-        if (v->uvs) std::memcpy(S + 18, v->uvs + (size_t)i * 6, 6 * sizeof(float));
-        int32_t obj = v->object_id[i];
-        int32_t mat = (obj >= 0 && (uint32_t)obj < v->mesh_count) ? v->mesh_material[obj] : 0;
-        if (mat < 0 || (uint32_t)mat >= v->material_count) mat = 0;
-        std::memcpy(&S[24], &mat, 4);
-        S[25] = v->tangent_sign ? v->tangent_sign[i] : 0.f;
-        if (v->tangents) std::memcpy(&tans[(size_t)i * dev::kTanFloats], v->tangents + (size_t)i * 9, 9 * sizeof(float));
-    }
+    parallelFor(v->tri_count, [&](size_t i0, size_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < (uint32_t)i1; i++) {
+            float* S = &shade[(size_t)i * dev::kShadeFloats];
+            std::memcpy(S, v->vertices + (size_t)i * 9, 9 * sizeof(float));
+            if (v->normals) std::memcpy(S + 9, v->normals + (size_t)i * 9, 9 * sizeof(float));
+            if (v->uvs) std::memcpy(S + 18, v->uvs + (size_t)i * 6, 6 * sizeof(float));
+            int32_t obj = v->object_id[i];
+            int32_t mat = (obj >= 0 && (uint32_t)obj < v->mesh_count) ? v->mesh_material[obj] : 0;
+            if (mat < 0 || (uint32_t)mat >= v->material_count) mat = 0;
+            std::memcpy(&S[24], &mat, 4);
+            S[25] = v->tangent_sign ? v->tangent_sign[i] : 0.f;
+            if (v->tangents) std::memcpy(&tans[(size_t)i * dev::kTanFloats], v->tangents + (size_t)i * 9, 9 * sizeof(float));
+        }
+    });
     EL_HIP(c, upload(c->dShade, shade.data(), shade.size() * sizeof(float), c->stream));
     EL_HIP(c, upload(c->dTangents, tans.data(), tans.size() * sizeof(float), c->stream));
 

@@ -1,4 +1,5 @@
 #include "hdri_table.hpp"
+#include "host_parallel.hpp"
 
 #include <algorithm>
 
@@ -41,10 +42,12 @@ HdriSampler buildHdriSampler(const float* rgb, int w, int h) {
     s.radianceSum = sum;
 
     s.map2.resize(2 * (size_t)L);
-    for (int j = 0; j < L; j++) {
-        s.map2[2 * (size_t)j] = hdriSearchIndex(j, L);
-        s.map2[2 * (size_t)j + 1] = hdriBinarySearch(s.cdf.data(), s.cdf[j], L);
-    }
+    parallelFor((size_t)L, [&](size_t j0, size_t j1) {
+        for (int j = (int)j0; j < (int)j1; j++) {
+            s.map2[2 * (size_t)j] = hdriSearchIndex(j, L);
+            s.map2[2 * (size_t)j + 1] = hdriBinarySearch(s.cdf.data(), s.cdf[j], L);
+        }
+    });
 
     int k = 0;
     while ((1 << k) < L) k++;
