@@ -187,7 +187,7 @@ Bvh2 buildBvh2(const float* v, int n, int threads) {
     out.order.resize((size_t)n);
     for (int i = 0; i < n; i++) out.order[i] = i;
     out.nodes.resize((size_t)std::max(1, 2 * n));
-    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads <= 0) threads = (int)hostThreads();
     int spawnDepth = 0;
     while ((1 << spawnDepth) < threads) spawnDepth++;
     Builder bld(prims, out.order, out.nodes, spawnDepth);

@@ -93,7 +93,8 @@ struct WideBvh {
     float sceneLo[3] = {0, 0, 0}, sceneHi[3] = {0, 0, 0};
 };
 
-// vertices: [n][3][3] floats.  Uses up to `threads` host threads (0 = all).
+// vertices: [n][3][3] floats.  Uses up to `threads` host threads (0 = the
+// hardware threads, capped by OMP_NUM_THREADS when it is set).
 Bvh2 buildBvh2(const float* vertices, int n, int threads = 0);
 // Collapse to width 4 or 8 and pack for the device; boxes are padded
 // outwards by `pad` (absolute, in scene units) so the slab test is
