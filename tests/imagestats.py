@@ -5,7 +5,7 @@ estimator as the reference R (the mean of n_ref samples, with per-sample
 variance v).  Under the hypothesis that both estimate the same expectation,
 each channel difference g - R has variance v / n + v / n_ref.
 
-relMSE (north_star's measure) = mean over pixels and channels of
+relMSE (relative mean squared error) = mean over pixels and channels of
 (g - R)^2 / (R^2 + 1e-2).  Its expectation under the hypothesis is the same
 mean of (v / n + v / n_ref) / (R^2 + 1e-2), and its standard deviation, with
 each difference Gaussian (n >= 256 samples) and the three channels of a pixel
