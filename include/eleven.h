@@ -6,9 +6,12 @@
  * (EL_OK == 0, negative on error) unless stated otherwise; the message of the
  * last error on a context is returned by el_last_error().
  *
- * Threading: one el_ctx per host thread; a context owns one HIP stream on one
- * device and starts no host threads of its own.  Calls are synchronous on
- * return unless stated otherwise.
+ * Threading: one el_ctx per host thread; a context owns three HIP streams
+ * on one device (its launches and the two shadow launches that may run
+ * beside them).  No host thread it starts outlives a call: el_upload_scene
+ * builds on up to the hardware's thread count, capped by OMP_NUM_THREADS
+ * when that is set.  Calls are synchronous on return unless stated
+ * otherwise.
  *
  * Reference interfaces replaced (paths relative to src/tfg-pathtracer/):
  *   renderSetup(Scene*)                       kernel.h:80,  kernel.cu:566-661
