@@ -1044,9 +1044,11 @@ __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, flo
 // nodes and leaves are separated (Aila & Laine, "Understanding the Efficiency
 // of Ray Traversal on GPUs", HPG 2009, speculative while-while): a lane that
 // reaches a leaf postpones it and keeps descending until every live lane of
-// the wave holds a leaf, then the wave tests leaves together.  The closest
-// hit is order-independent (t, then lower index), so results are identical
-// to traceRay.
+// the wave holds a leaf, then the wave tests leaves together.  A lane may
+// postpone a second leaf and go on descending instead of idling while the
+// other lanes look for their first (measured: a third does not pay).  The
+// closest hit is order-independent (t, then lower index), so results are
+// identical to traceRay.
 //
 //   load(i, ox, oy, oz, dx, dy, dz, tmax, bvh) -> false if item i has no ray;
 //                                   with PER_RAY_BVH it also sets the BVH
@@ -1113,7 +1115,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
     uint32_t item = 0;
     TRay ray{};
     TraceResult r{};
-    int cur = -1, leaf = -1;
+    int cur = -1, leaf = -1, leaf2 = -1;
     while (true) {
         // ---- refill idle lanes from the wave's range or the next chunk
         const unsigned long long idle = __ballot(!live);
@@ -1191,7 +1193,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                         r.u = r.v = 0.f;
                         st.sp = 0;
                         st.lo = 0;
-                        leaf = -1;
+                        leaf = leaf2 = -1;
                         cur = 0;
                         if constexpr (!PER_RAY_BVH) {
                             cur = visitWords<W, ANY>(root, ray, cullT<ANY>(r.t), st);
@@ -1225,30 +1227,36 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                 if (cur <= -2 && leaf == -1) {
                     leaf = cur;
                     cur = st.pop();
+                } else if (cur <= -2 && leaf2 == -1) {   // a second leaf: postponed as well
+                    leaf2 = cur;
+                    cur = st.pop();
                 }
             }
         }
         // ---- leaves: the postponed one, then any leaves that follow on the stack
         while (true) {
-            const bool has = live && (leaf != -1 || cur <= -2);
+            const bool has = live && (leaf != -1 || leaf2 != -1 || cur <= -2);
             if (!__any(has)) break;
             if (has) {
                 int code;
                 if (leaf != -1) {
                     code = leaf;
                     leaf = -1;
+                } else if (leaf2 != -1) {
+                    code = leaf2;
+                    leaf2 = -1;
                 } else {
                     code = cur;
                     cur = st.pop();
                 }
                 if (testLeaf<ANY, COUNT>(PER_RAY_BVH ? bvh.tris : s.tris, code, ray, r, cnt)) {
                     cur = -1;   // any-hit: done
-                    leaf = -1;
+                    leaf = leaf2 = -1;
                 }
             }
         }
         // ---- finished rays
-        if (live && cur == -1 && leaf == -1) {
+        if (live && cur == -1 && leaf == -1 && leaf2 == -1) {
             store(item, r, true);
             traced++;
             live = false;
