@@ -1044,9 +1044,11 @@ __device__ __forceinline__ TraceResult traceRay(const DevScene& s, float ox, flo
 // nodes and leaves are separated (Aila & Laine, "Understanding the Efficiency
 // of Ray Traversal on GPUs", HPG 2009, speculative while-while): a lane that
 // reaches a leaf postpones it and keeps descending until every live lane of
-// the wave holds a leaf, then the wave tests leaves together.  A lane may
-// postpone a second leaf and go on descending instead of idling while the
-// other lanes look for their first (measured: a third does not pay).  The
+// the wave holds a leaf, then the wave tests leaves together.  With one BVH
+// per wave a lane may postpone a second leaf and go on descending instead
+// of idling while the other lanes look for their first (measured: a third
+// does not pay, nor does a second with per-ray BVHs, whose lanes hold more
+// registers).  The
 // closest hit is order-independent (t, then lower index), so results are
 // identical to traceRay.
 //
@@ -1227,7 +1229,7 @@ __device__ __forceinline__ void traceWaveRange(const DevScene& s, TravStack& st,
                 if (cur <= -2 && leaf == -1) {
                     leaf = cur;
                     cur = st.pop();
-                } else if (cur <= -2 && leaf2 == -1) {   // a second leaf: postponed as well
+                } else if (!PER_RAY_BVH && cur <= -2 && leaf2 == -1) {   // a second leaf: postponed as well
                     leaf2 = cur;
                     cur = st.pop();
                 }
